@@ -1,0 +1,173 @@
+"""Throughput of the nuosc RK4 time-stepper core on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision 64|32]
+                    [--nn 8] [--zmax 12.8] [--dz 0.05] [--no-cpu]
+
+Workload (default): the 512 x 512 periodic box with 642 icosahedral
+directions (--zmax 12.8 --dz 0.05 --nv 8), FP64, mu=1, ko=0.1, cfl=0.4,
+alpha=0.9, sigma_nu=0.6, sigma_nubar=0.5, seeded eps=1e-6 complex
+perturbation.  A step is one full RK4 step (four stage launches, replayed
+from a hipGraph) over the whole state, which stays resident in HBM.
+
+Prints one JSON line:
+  value          site*dir RK4 updates/s = nx*nz*nv*K / wall(K steps)
+  roofline       for the dominant kernel (RK4 stage 2): algorithmic bytes per
+                 launch (5 state passes x 8 fields x sizeof(real) per update)
+                 / its average duration from HIP events, against 8 TB/s
+  step_roofline  the same for a whole step (1024 B per update in FP64)
+  cpu_baseline   the CPU oracle on a 128 x 128 x 642 crop of the same physics,
+                 timed on this host's cores (a reported baseline)
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+# state passes per update of each stage (read + write of the 8 fields)
+STAGE_PASSES = (3, 5, 5, 3)
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=int(os.environ.get("WORLD_SIZE", "1")))
+    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--precision", type=int, default=64, choices=(64, 32))
+    p.add_argument("--nn", type=int, default=8)
+    p.add_argument("--zmax", type=float, default=12.8)
+    p.add_argument("--dz", type=float, default=0.05)
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--cpu-seconds", type=float, default=12.0)
+    return p.parse_args()
+
+
+def traffic_from_profiles(precision):
+    """HBM bytes per stage-2 launch measured with rocprofv3 --pmc (FETCH_SIZE +
+    WRITE_SIZE, scaled by a copy-kernel calibration), if recorded."""
+    path = os.path.join(ROOT, "profiles", "stage_traffic.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        return d.get(f"fp{precision}", {}).get("stage2_bytes")
+    except Exception:
+        return None
+
+
+def cpu_baseline(nn, dz, seconds):
+    """Oracle (OpenMP C restatement of the reference) on a 128x128 crop."""
+    from oracle import oracle as O
+    if not os.path.exists(O.LIB_PATH):
+        O.build()
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+    half = 64 * dz
+    o = O.OracleSim(nn=nn, x0=-half, x1=half, z0=-half, z1=half, dx=dz, dz=dz, cfl=0.4, ko=0.1, mu=1.0, pmo=0.0)
+    o.fill_init(ipt=5, alpha=0.9, eps0=1e-6, sigma=1.0, lnue=0.6, lnueb=0.5, seed=12345)
+    nx, nz, nv = o.shape
+    o.step(1)  # warm caches / page in
+    n, t0 = 0, time.perf_counter()
+    while True:
+        o.step(1)
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= seconds or n >= 50:
+            break
+    o.close()
+    return {
+        "value": nx * nz * nv * n / el,
+        "unit": "site*dir RK4 updates/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{nx}x{nz}x{nv} crop of the same physics (FP64), {n} RK4 steps in {el:.1f} s, "
+                  f"OMP_NUM_THREADS={threads}",
+    }
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {a.gpus}); "
+              "the periodic box would shard as an (x,z) domain decomposition with radius-2 halos.",
+              file=sys.stderr)
+        sys.exit(2)
+    import nuosc_amd as N
+
+    prec = N.PREC_FP64 if a.precision == 64 else N.PREC_FP32
+    esize = 8 if a.precision == 64 else 4
+    cfg = dict(nv=a.nn, x0=-a.zmax, x1=a.zmax, z0=-a.zmax, z1=a.zmax, dx=a.dz, dz=a.dz, CFL=0.4, ko=0.1,
+               mu=1.0, pmo=0.0, precision=prec, use_graph=True)
+    g = N.NuOsc(device=0, **cfg)
+    g.fillInitValue(ipt=5, alpha=0.9, eps0=1e-6, sigma=1.0, lnue=0.6, lnueb=0.5, seed=12345)
+    nx, nz, nv = g.shape
+    upd = nx * nz * nv
+
+    # per-stage kernel durations from HIP events on the library's stream
+    stage_ms = g.profile_stages(2)
+    g.step_rk4(a.warmup)
+    g.synchronize()
+    t0 = time.perf_counter()
+    g.step_rk4(a.steps)
+    g.synchronize()
+    wall = time.perf_counter() - t0
+    dev_ms = g.last_step_ms()
+    ana = g.analysis()
+    ok = ana.status == 0 and ana.nonfinite == 0
+    g.close()
+
+    value = upd * a.steps / wall
+    s2_bytes = STAGE_PASSES[1] * 8 * esize * upd
+    s2_gbs = s2_bytes / (stage_ms[1] * 1e-3) / 1e9
+    step_bytes = sum(STAGE_PASSES) * 8 * esize * upd
+    step_gbs = step_bytes / (wall / a.steps) / 1e9
+    out = {
+        "metric": "site*dir RK4 updates/s",
+        "value": value,
+        "unit": "updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": wall / a.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if a.precision == 64 else "f32",
+        "data": "synthetic (seeded eps=1e-6 complex perturbation)",
+        "config": {
+            "workload": f"{nx}x{nz} periodic box x {nv} icosahedral directions, "
+                        f"{'FP64' if a.precision == 64 else 'FP32 + FP64 moments'}, mu=1 ko=0.1 cfl=0.4 pmo=0",
+            "nx": nx, "nz": nz, "nv": nv, "dz": a.dz,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_stage stage 2",
+            "achieved": s2_gbs,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": s2_gbs / HBM_PEAK_GBS,
+            "traffic": traffic_from_profiles(a.precision),
+            "algorithmic_bytes_per_launch": s2_bytes,
+            "launch_ms": stage_ms[1],
+        },
+        "step_roofline": {
+            "achieved": step_gbs,
+            "frac": step_gbs / HBM_PEAK_GBS,
+            "bytes_per_update": sum(STAGE_PASSES) * 8 * esize,
+            "stage_ms": [round(x, 4) for x in stage_ms],
+            "device_ms_per_step": dev_ms / a.steps,
+        },
+        "state_ok": ok,
+    }
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(a.nn, a.dz, a.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

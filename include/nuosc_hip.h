@@ -1,0 +1,184 @@
+/*
+ * nuosc_hip.h -- C ABI of libnuosc_hip.so, the MI355X (gfx950) implementation
+ * of the nuosc RK4 time-stepper core.
+ *
+ * The reference exposes this path through the C++ class NuOsc
+ * (nuosc_class.h:188-280) and the ./nuosc command line (nuosc.cpp:17-261); it
+ * has no plugin or FFI layer.  This header is the plain-C boundary beneath a
+ * NuOsc-compatible host class (nuosc_amd/csrc/NuOsc.h) and the ctypes binding
+ * (nuosc_amd/__init__.py).  Every entry point below cites the reference
+ * member it replaces.
+ *
+ * Conventions
+ *   - A handle (nuosc_sim*) owns all device memory; it is driven by one host
+ *     thread and uses one HIP stream.  Distinct handles are independent.
+ *   - Every function returns 0 on success or a nonzero NUOSC_E* code; the
+ *     message is available from nuosc_last_error(handle) (or
+ *     nuosc_last_error(NULL) for errors of nuosc_create).  No C++ exception
+ *     crosses this boundary.
+ *   - Host arrays are FP64.  Field order is the reference's
+ *     (nuosc_class.h:147-153): 0 ee, 1 mm (=xx), 2 emr, 3 emi, 4 bee, 5 bmm,
+ *     6 bemr, 7 bemi.  The "interior" layout is [nx][nz][nv] (v fastest); the
+ *     "ghosted" layout is the reference's lpts layout
+ *     ((i+gx)*(nz+2gz)+j+gz)*nv+v (CartGrid.h:91).
+ */
+#ifndef NUOSC_HIP_H
+#define NUOSC_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NUOSC_ABI_VERSION 1
+
+/* status codes */
+#define NUOSC_OK 0
+#define NUOSC_EINVAL 1      /* bad argument or configuration            */
+#define NUOSC_EHIP 2        /* a HIP runtime call failed                */
+#define NUOSC_ENOMEM 3      /* device or host allocation failed         */
+#define NUOSC_EUNSTABLE 4   /* analysis: maxdP >= 10 or non-finite state */
+#define NUOSC_ENODEV 5      /* no usable GPU                            */
+
+#define NUOSC_PREC_FP64 64
+#define NUOSC_PREC_FP32 32
+
+#define NUOSC_VGRID_VORONOI 0 /* icosahedron.h weights (reference)          */
+#define NUOSC_VGRID_EQUAL 1   /* same directions, weights 4*pi/N            */
+
+#define NUOSC_BC_PERIODIC 0
+
+typedef struct nuosc_sim nuosc_sim;
+
+/* Run configuration.  Replaces the NuOsc constructor arguments
+ * NuOsc(px,pz,nv,nphi,gx,gz,x0,x1,z0,z1,dx,dz,CFL,ko) (nuosc_class.h:220-222,
+ * nuosc_class.cpp:454-466) plus set_mu/set_pmo/set_renorm
+ * (nuosc_class.h:237-248).  px,pz (MPI) and nphi (polar grid only) have no
+ * counterpart here. */
+typedef struct nuosc_config {
+    int nn;          /* icosahedron refinement (--nv): N = 10*nn^2+2 directions */
+    int gx, gz;      /* ghost widths of the reference layout (2 by default) */
+    double x0, x1, z0, z1, dx, dz;
+    double cfl;      /* dt = dz*cfl (nuosc_class.cpp:464) */
+    double ko;       /* Kreiss-Oliger coefficient */
+    double mu;       /* self-interaction strength */
+    double pmo;      /* vacuum term: 1/-1 mass ordering, 0 off */
+    int renorm;      /* renormalise |P| after each step */
+    int vgrid;       /* NUOSC_VGRID_* */
+    int precision;   /* NUOSC_PREC_FP64 or NUOSC_PREC_FP32 */
+    int bc;          /* NUOSC_BC_PERIODIC */
+    int device;      /* HIP device ordinal */
+    int use_graph;   /* capture one step as a hipGraph and replay it */
+} nuosc_config;
+
+/* Diagnostics of one analysis() call (nuosc_ana.cpp:34-137).  The first nine
+ * are the analysis.dat columns; the rest are additions. */
+typedef struct nuosc_analysis_t {
+    double phy_time;
+    double maxdP;    /* max |1-|P|| over nu and nubar                   */
+    double surv;     /* Pee:    sum w ee / sum w G0                     */
+    double survb;    /* Peebar: sum w bee / sum w G0b                   */
+    double avgP;     /* sum w G0 |1-|P|| / sum w G0                     */
+    double avgPb;
+    double aM0;      /* |M0| * ds_L                                     */
+    double Lex;      /* M0_3 * ds_L                                     */
+    double ELNe;
+    /* additions (not written to analysis.dat) */
+    double rho_ex2;  /* sum w (emr^2 + emi^2) * ds_L: |rho_ex| growth   */
+    double nonfinite;/* number of points with a non-finite field value  */
+    int status;      /* 0, or NUOSC_EUNSTABLE                            */
+} nuosc_analysis_t;
+
+/* Library identity. */
+int nuosc_abi_version(void);
+const char* nuosc_build_info(void);
+
+/* Host-only (no GPU call): the velocity grid a handle would use, as
+ * gen_v2d_icosahedron (CartGrid.cpp:46-61) builds it.  mode is NUOSC_VGRID_*.
+ * Arrays must hold 10*nn^2+2 doubles; returns that count, or -1. */
+int nuosc_vgrid(int nn, int mode, double* vx, double* vy, double* vz, double* vw);
+
+/* NuOsc::NuOsc (nuosc_class.cpp:454-534): geometry, velocity grid,
+ * allocation.  The state is zero until nuosc_fill_init or an upload. */
+int nuosc_create(const nuosc_config* cfg, nuosc_sim** out);
+/* NuOsc::~NuOsc (nuosc_class.h:225-235) */
+int nuosc_destroy(nuosc_sim* sim);
+const char* nuosc_last_error(const nuosc_sim* sim);
+
+/* NuOsc::set_mu / set_pmo / set_renorm (nuosc_class.h:237-248) */
+int nuosc_set_mu(nuosc_sim* sim, double mu);
+int nuosc_set_pmo(nuosc_sim* sim, double pmo);
+int nuosc_set_renorm(nuosc_sim* sim, int renorm);
+
+/* Grid queries: CartGrid members nx, nz, nv, lpts (CartGrid.h:27-34,85-89)
+ * and NuOsc members dt, ds_L, phy_time, n_nue0 (nuosc_class.h:192-203). */
+int nuosc_get_dims(const nuosc_sim* sim, int* nx, int* nz, int* nv, long long* lpts);
+int nuosc_get_scalars(const nuosc_sim* sim, double* dt, double* ds_L, double* phy_time,
+                      double* n_nue0_2);
+int nuosc_set_phy_time(nuosc_sim* sim, double phy_time);
+/* CartGrid vx, vy, vz, vw (length nv) and X (nx), Z (nz) (CartGrid.h:29-32) */
+int nuosc_get_vgrid(const nuosc_sim* sim, double* vx, double* vy, double* vz, double* vw);
+int nuosc_get_coords(const nuosc_sim* sim, double* X, double* Z);
+/* G0, G0b per direction (the reference stores them per point,
+ * nuosc_class.h:202). */
+int nuosc_get_g0(const nuosc_sim* sim, double* G0, double* G0b);
+/* Device bytes held by the handle. */
+int nuosc_get_device_bytes(const nuosc_sim* sim, long long* bytes);
+
+/* NuOsc::fillInitValue(ipt,alpha,eps0,sigma,lnue,lnueb,lnuex,lnuebx)
+ * (nuosc_init.cpp:17-137) for ipt 0, 2, 3; ipt 5 adds a seeded complex
+ * perturbation rho_ex = eps0*(ur + i ui)*G0/2 with (ur, ui) uniform in [-1,1)
+ * from a counter-based generator keyed by (seed, interior index).  ipt 1 and
+ * 4 use the C library rand() in the reference and are rejected. */
+int nuosc_fill_init(nuosc_sim* sim, int ipt, double alpha, double eps0, double sigma, double lnue,
+                    double lnueb, double lnuex, double lnuebx, uint64_t seed);
+
+/* NuOsc::step_rk4 (nuosc_class.cpp:374-427), nsteps times; phy_time += dt
+ * per step.  Asynchronous on the handle's stream. */
+int nuosc_step_rk4(nuosc_sim* sim, int nsteps);
+int nuosc_synchronize(nuosc_sim* sim);
+
+/* NuOsc::calRHS(out, in) (nuosc_class.cpp:11-307) with the periodic ghost
+ * fill of its input (nuosc_boundary.cpp:4-52): in/out are host arrays of
+ * 8*nx*nz*nv doubles in the interior layout.  Test primitive; uses the
+ * handle's scratch state buffers (the state itself is left untouched). */
+int nuosc_calrhs(nuosc_sim* sim, const double* in, double* out);
+
+/* NuOsc::analysis (nuosc_ana.cpp:34-137) on the current state. */
+int nuosc_analysis(nuosc_sim* sim, nuosc_analysis_t* out);
+
+/* NuOsc::renormalize (nuosc_ana.cpp:140-166) on the current state. */
+int nuosc_renormalize(nuosc_sim* sim);
+
+/* State transfer (replaces direct access to v_stat->ee etc.).
+ * download_field: one field, interior layout (nx*nz*nv doubles).
+ * download_field_ghosted: one field in the reference's ghosted layout with
+ *   the ghost band filled by updatePeriodicBoundary's rule
+ *   (nuosc_boundary.cpp:4-52) and zero corners ((nx+2gx)(nz+2gz)nv doubles).
+ * upload_field: one field from the interior layout.  After uploading all the
+ *   fields call nuosc_state_updated to refresh the velocity moments. */
+int nuosc_download_field(nuosc_sim* sim, int field, double* host);
+int nuosc_download_field_ghosted(nuosc_sim* sim, int field, double* host);
+int nuosc_upload_field(nuosc_sim* sim, int field, const double* host);
+int nuosc_state_updated(nuosc_sim* sim);
+
+/* P3 snapshot slice (nuosc_snapshot.cpp:82-122 with var = P3, refreshed as
+ * in eval_conserved, nuosc_ana.cpp:14): dst[sx][nz][sv] with
+ * P3 = (ee-mm)/G0 at x indices xidx[sx] and direction indices vidx[sv]. */
+int nuosc_p3_slice(nuosc_sim* sim, const int* xidx, int sx, const int* vidx, int sv, double* dst);
+
+/* Wall-clock of the device work of the last nuosc_step_rk4 call (ms),
+ * from HIP events on the handle's stream (synchronises). */
+int nuosc_last_step_ms(nuosc_sim* sim, double* ms);
+
+/* Measurement helper: run nsteps RK4 steps eagerly with HIP events between
+ * the four stage launches (on the handle's stream) and return the average
+ * duration of each stage kernel in stage_ms[4] (ms).  Advances the state and
+ * phy_time like nuosc_step_rk4. */
+int nuosc_profile_stages(nuosc_sim* sim, int nsteps, double* stage_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NUOSC_HIP_H */

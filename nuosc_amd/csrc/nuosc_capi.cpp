@@ -1,0 +1,659 @@
+// nuosc_capi.cpp -- the extern "C" boundary (include/nuosc_hip.h): handle
+// lifetime, device memory, the per-step launch sequence (optionally replayed
+// from a hipGraph) and host<->device transfers.  Host C++; kernels live in
+// nuosc_kernels.hip.
+#include "../../include/nuosc_hip.h"
+
+#include <hip/hip_runtime_api.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "nuosc_dev.h"
+#include "vgrid.h"
+
+using namespace nuosc;
+
+namespace {
+thread_local std::string g_create_error;
+}
+
+struct nuosc_sim {
+    nuosc_config cfg{};
+    // geometry (CartGrid.cpp:191-208)
+    int nx = 0, nz = 0, nv = 0, nvp = 0, nch = 0;
+    long long fs = 0;
+    double dt = 0, ds_L = 0, ct = 0, st = 0;
+    double phy_time = 0;
+    double n_nue0[2] = {0, 0};
+    std::vector<double> X, Z;
+    VGrid vg;
+    std::vector<double> G0h, G0bh;
+    size_t esize = 8;  // bytes per stored value
+    // device memory
+    void* st_u = nullptr;
+    void* st_w = nullptr;
+    void* st_ya = nullptr;
+    void* st_yb = nullptr;
+    double* mom_u = nullptr;
+    double* mom_a = nullptr;
+    double* mom_b = nullptr;
+    double* d_vgrid = nullptr;  // vx | vy | vz | vw
+    double* d_g0 = nullptr;     // G0 | G0b
+    double* d_part = nullptr;
+    double* d_ana = nullptr;
+    double* d_tmp = nullptr;    // FP64 staging (interior or ghosted field)
+    size_t tmp_bytes = 0;
+    long long device_bytes = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    bool have_moments = false;
+    int steps_in_last_call = 0;
+    Geometry geo{};
+    Physics ph{};
+    StagePlan plan{};
+    std::string err;
+};
+
+namespace {
+
+int fail(nuosc_sim* s, int code, const std::string& msg) {
+    if (s) s->err = msg;
+    else g_create_error = msg;
+    return code;
+}
+
+#define HIPCK(sim, call)                                                                      \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return fail((sim), NUOSC_EHIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+bool is64(const nuosc_sim* s) { return s->cfg.precision == NUOSC_PREC_FP64; }
+
+template <typename T>
+T* field_ptr(void* base, const nuosc_sim* s, int f) {
+    return static_cast<T*>(base) + (long long)f * s->fs;
+}
+
+void drop_graph(nuosc_sim* s) {
+    if (s->graph_exec) hipGraphExecDestroy(s->graph_exec);
+    if (s->graph) hipGraphDestroy(s->graph);
+    s->graph_exec = nullptr;
+    s->graph = nullptr;
+}
+
+int ensure_tmp(nuosc_sim* s, size_t bytes) {
+    if (s->tmp_bytes >= bytes) return NUOSC_OK;
+    if (s->d_tmp) {
+        hipStreamSynchronize(s->stream);
+        hipFree(s->d_tmp);
+        s->device_bytes -= (long long)s->tmp_bytes;
+        s->d_tmp = nullptr;
+        s->tmp_bytes = 0;
+    }
+    if (hipMalloc(&s->d_tmp, bytes) != hipSuccess) return fail(s, NUOSC_ENOMEM, "staging buffer allocation failed");
+    s->tmp_bytes = bytes;
+    s->device_bytes += (long long)bytes;
+    return NUOSC_OK;
+}
+
+// One RK4 step: the four stage launches (+ optional renormalisation).  With
+// ev != nullptr an event is recorded before each stage and after the last.
+template <typename T>
+int enqueue_step(nuosc_sim* s, hipEvent_t* ev = nullptr) {
+    const double dt = s->dt;
+    const double c6 = 1.0 / 6.0 * dt, c3 = 1.0 / 3.0 * dt, c2 = 0.5 * dt;
+    T* u = static_cast<T*>(s->st_u);
+    T* w = static_cast<T*>(s->st_w);
+    T* ya = static_cast<T*>(s->st_ya);
+    T* yb = static_cast<T*>(s->st_yb);
+    const Geometry& g = s->geo;
+    if (ev) HIPCK(s, hipEventRecord(ev[0], s->stream));
+    HIPCK(s, launch_stage<T>(1, g, s->ph, s->plan, u, nullptr, w, ya, s->mom_u, s->mom_a, c6, c2, s->stream));
+    if (ev) HIPCK(s, hipEventRecord(ev[1], s->stream));
+    HIPCK(s, launch_stage<T>(2, g, s->ph, s->plan, ya, u, w, yb, s->mom_a, s->mom_b, c3, c2, s->stream));
+    if (ev) HIPCK(s, hipEventRecord(ev[2], s->stream));
+    HIPCK(s, launch_stage<T>(3, g, s->ph, s->plan, yb, u, w, ya, s->mom_b, s->mom_a, c3, dt, s->stream));
+    if (ev) HIPCK(s, hipEventRecord(ev[3], s->stream));
+    HIPCK(s, launch_stage<T>(4, g, s->ph, s->plan, ya, nullptr, w, u, s->mom_a, s->mom_u, c6, 0.0, s->stream));
+    if (ev) HIPCK(s, hipEventRecord(ev[4], s->stream));
+    if (s->cfg.renorm) {
+        HIPCK(s, launch_renorm<T>(g, u, s->d_g0, s->d_g0 + s->nv, s->stream));
+        HIPCK(s, launch_moments<T>(g, u, s->mom_u, s->stream));
+    }
+    return NUOSC_OK;
+}
+
+int refresh_moments(nuosc_sim* s) {
+    if (is64(s)) HIPCK(s, launch_moments<double>(s->geo, (double*)s->st_u, s->mom_u, s->stream));
+    else HIPCK(s, launch_moments<float>(s->geo, (float*)s->st_u, s->mom_u, s->stream));
+    s->have_moments = true;
+    return NUOSC_OK;
+}
+
+// Device analysis sums -> host (kNAna doubles).
+int analysis_sums(nuosc_sim* s, double* h) {
+    if (is64(s))
+        HIPCK(s, launch_analysis<double>(s->geo, (double*)s->st_u, s->d_g0, s->d_g0 + s->nv, s->d_part, s->d_ana,
+                                         s->stream));
+    else
+        HIPCK(s, launch_analysis<float>(s->geo, (float*)s->st_u, s->d_g0, s->d_g0 + s->nv, s->d_part, s->d_ana,
+                                        s->stream));
+    HIPCK(s, hipMemcpyAsync(h, s->d_ana, sizeof(double) * kNAna, hipMemcpyDeviceToHost, s->stream));
+    HIPCK(s, hipStreamSynchronize(s->stream));
+    return NUOSC_OK;
+}
+
+int check_field(nuosc_sim* s, int f) {
+    if (f < 0 || f >= kNVar) return fail(s, NUOSC_EINVAL, "field index out of range");
+    return NUOSC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int nuosc_abi_version(void) { return NUOSC_ABI_VERSION; }
+
+const char* nuosc_build_info(void) {
+    return "libnuosc_hip: gfx950 RK4 stage kernels (fp64, fp32 + fp64 moments), abi " "1";
+}
+
+int nuosc_vgrid(int nn, int mode, double* vx, double* vy, double* vz, double* vw) {
+    if (nn < 1 || nn > 64 || !vx || !vy || !vz || !vw) return -1;
+    if (mode != NUOSC_VGRID_VORONOI && mode != NUOSC_VGRID_EQUAL) return -1;
+    const VGrid g = (mode == NUOSC_VGRID_EQUAL) ? make_equal_weight_vgrid(nn) : make_icosahedral_vgrid(nn);
+    const size_t b = sizeof(double) * g.size();
+    std::memcpy(vx, g.vx.data(), b);
+    std::memcpy(vy, g.vy.data(), b);
+    std::memcpy(vz, g.vz.data(), b);
+    std::memcpy(vw, g.vw.data(), b);
+    return g.size();
+}
+
+const char* nuosc_last_error(const nuosc_sim* s) { return s ? s->err.c_str() : g_create_error.c_str(); }
+
+int nuosc_create(const nuosc_config* cfg, nuosc_sim** out) {
+    if (!cfg || !out) return fail(nullptr, NUOSC_EINVAL, "null argument");
+    *out = nullptr;
+    const nuosc_config c = *cfg;
+    if (c.nn < 1 || c.nn > 64) return fail(nullptr, NUOSC_EINVAL, "nn must be in [1, 64]");
+    if (c.gx < 2 || c.gz < 2 || c.gx > 8 || c.gz > 8) return fail(nullptr, NUOSC_EINVAL, "ghost widths must be in [2, 8]");
+    if (!(c.dx > 0) || !(c.dz > 0) || !(c.x1 > c.x0) || !(c.z1 > c.z0))
+        return fail(nullptr, NUOSC_EINVAL, "bad box or spacing");
+    if (c.precision != NUOSC_PREC_FP64 && c.precision != NUOSC_PREC_FP32)
+        return fail(nullptr, NUOSC_EINVAL, "precision must be 64 or 32");
+    if (c.bc != NUOSC_BC_PERIODIC) return fail(nullptr, NUOSC_EINVAL, "only periodic boundaries are supported");
+    if (c.vgrid != NUOSC_VGRID_VORONOI && c.vgrid != NUOSC_VGRID_EQUAL)
+        return fail(nullptr, NUOSC_EINVAL, "unknown velocity grid");
+
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(nullptr, NUOSC_ENODEV, "no HIP device");
+    if (c.device < 0 || c.device >= ndev) return fail(nullptr, NUOSC_EINVAL, "device ordinal out of range");
+
+    nuosc_sim* s = new nuosc_sim();
+    s->cfg = c;
+    // CartGrid.cpp:197-201
+    s->nx = int((c.x1 - c.x0) / c.dx);
+    s->nz = int((c.z1 - c.z0) / c.dz);
+    if (s->nx < 1 || s->nz < 1) {
+        delete s;
+        return fail(nullptr, NUOSC_EINVAL, "empty grid");
+    }
+    s->X.resize(s->nx);
+    s->Z.resize(s->nz);
+    for (int i = 0; i < s->nx; ++i) s->X[i] = c.x0 + (i + 0.5) * c.dx;
+    for (int i = 0; i < s->nz; ++i) s->Z[i] = c.z0 + (i + 0.5) * c.dz;
+    s->vg = (c.vgrid == NUOSC_VGRID_EQUAL) ? make_equal_weight_vgrid(c.nn) : make_icosahedral_vgrid(c.nn);
+    s->nv = s->vg.size();
+    s->esize = is64(s) ? 8 : 4;
+    const int align = (int)(128 / s->esize);
+    s->nvp = (s->nv + align - 1) / align * align;
+    s->nch = (s->nv + kWave - 1) / kWave;
+    s->fs = (long long)s->nx * s->nz * s->nvp;
+    // nuosc_class.cpp:461-464, nuosc_class.h:120-122
+    s->ds_L = c.dx * c.dz / (c.z1 - c.z0) / (c.x1 - c.x0);
+    s->dt = c.dz * c.cfl;
+    const double theta = 37 * M_PI / 180.;
+    s->ct = std::cos(2 * theta);
+    s->st = std::sin(2 * theta);
+    s->ph = Physics{c.mu, c.pmo, s->ct, s->st};
+
+    auto bail = [&](int code, const std::string& m) {
+        std::string msg = m;
+        nuosc_destroy(s);
+        return fail(nullptr, code, msg);
+    };
+    if (hipSetDevice(c.device) != hipSuccess) return bail(NUOSC_EHIP, "hipSetDevice failed");
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(NUOSC_EHIP, "stream creation failed");
+    if (hipEventCreate(&s->ev0) != hipSuccess || hipEventCreate(&s->ev1) != hipSuccess)
+        return bail(NUOSC_EHIP, "event creation failed");
+
+    const size_t state_bytes = (size_t)kNVar * s->fs * s->esize;
+    const size_t mom_bytes = (size_t)s->nx * s->nz * s->nch * kMomStride * sizeof(double);
+    void** states[4] = {&s->st_u, &s->st_w, &s->st_ya, &s->st_yb};
+    for (void** p : states) {
+        if (hipMalloc(p, state_bytes) != hipSuccess) return bail(NUOSC_ENOMEM, "state allocation failed");
+        s->device_bytes += (long long)state_bytes;
+        if (hipMemsetAsync(*p, 0, state_bytes, s->stream) != hipSuccess) return bail(NUOSC_EHIP, "memset failed");
+    }
+    double** moms[3] = {&s->mom_u, &s->mom_a, &s->mom_b};
+    for (double** p : moms) {
+        if (hipMalloc((void**)p, mom_bytes) != hipSuccess) return bail(NUOSC_ENOMEM, "moment allocation failed");
+        s->device_bytes += (long long)mom_bytes;
+        if (hipMemsetAsync(*p, 0, mom_bytes, s->stream) != hipSuccess) return bail(NUOSC_EHIP, "memset failed");
+    }
+    const size_t vbytes = 4 * (size_t)s->nv * sizeof(double);
+    if (hipMalloc((void**)&s->d_vgrid, vbytes) != hipSuccess) return bail(NUOSC_ENOMEM, "allocation failed");
+    if (hipMalloc((void**)&s->d_g0, 2 * s->nv * sizeof(double)) != hipSuccess)
+        return bail(NUOSC_ENOMEM, "allocation failed");
+    s->geo.nx = s->nx;
+    s->geo.nz = s->nz;
+    s->geo.nv = s->nv;
+    s->geo.nvp = s->nvp;
+    s->geo.nch = s->nch;
+    s->geo.fs = s->fs;
+    s->geo.twelve_dx = 12 * c.dx;
+    s->geo.twelve_dz = 12 * c.dz;
+    s->geo.ko_x = -c.ko / c.dx / 16.0;  // nuosc_class.cpp:120-121
+    s->geo.ko_z = -c.ko / c.dz / 16.0;
+    s->geo.vx = s->d_vgrid;
+    s->geo.vy = s->d_vgrid + s->nv;
+    s->geo.vz = s->d_vgrid + 2 * s->nv;
+    s->geo.vw = s->d_vgrid + 3 * s->nv;
+    s->geo.bc = c.bc;
+    const int nb = ana_blocks(s->geo);
+    if (hipMalloc((void**)&s->d_part, (size_t)nb * kNAna * 2 * sizeof(double)) != hipSuccess ||
+        hipMalloc((void**)&s->d_ana, kNAna * sizeof(double)) != hipSuccess)
+        return bail(NUOSC_ENOMEM, "allocation failed");
+    std::vector<double> hv(4 * (size_t)s->nv);
+    for (int k = 0; k < s->nv; ++k) {
+        hv[k] = s->vg.vx[k];
+        hv[s->nv + k] = s->vg.vy[k];
+        hv[2 * s->nv + k] = s->vg.vz[k];
+        hv[3 * s->nv + k] = s->vg.vw[k];
+    }
+    if (hipMemcpyAsync(s->d_vgrid, hv.data(), vbytes, hipMemcpyHostToDevice, s->stream) != hipSuccess)
+        return bail(NUOSC_EHIP, "copy failed");
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, c.device) != hipSuccess) return bail(NUOSC_EHIP, "device query failed");
+    s->plan = plan_stage(s->geo, prop.multiProcessorCount);
+    if (hipStreamSynchronize(s->stream) != hipSuccess) return bail(NUOSC_EHIP, "stream sync failed");
+    s->G0h.assign(s->nv, 0.0);
+    s->G0bh.assign(s->nv, 0.0);
+    *out = s;
+    return NUOSC_OK;
+}
+
+int nuosc_destroy(nuosc_sim* s) {
+    if (!s) return NUOSC_OK;
+    if (s->stream) hipStreamSynchronize(s->stream);
+    drop_graph(s);
+    void* bufs[] = {s->st_u, s->st_w, s->st_ya, s->st_yb, s->mom_u, s->mom_a, s->mom_b,
+                    s->d_vgrid, s->d_g0, s->d_part, s->d_ana, s->d_tmp};
+    for (void* p : bufs)
+        if (p) hipFree(p);
+    if (s->ev0) hipEventDestroy(s->ev0);
+    if (s->ev1) hipEventDestroy(s->ev1);
+    if (s->stream) hipStreamDestroy(s->stream);
+    delete s;
+    return NUOSC_OK;
+}
+
+int nuosc_set_mu(nuosc_sim* s, double mu) {
+    if (!s) return NUOSC_EINVAL;
+    s->cfg.mu = mu;
+    s->ph.mu = mu;
+    drop_graph(s);
+    return NUOSC_OK;
+}
+
+int nuosc_set_pmo(nuosc_sim* s, double pmo) {
+    if (!s) return NUOSC_EINVAL;
+    s->cfg.pmo = pmo;
+    s->ph.pmo = pmo;
+    drop_graph(s);
+    return NUOSC_OK;
+}
+
+int nuosc_set_renorm(nuosc_sim* s, int renorm) {
+    if (!s) return NUOSC_EINVAL;
+    s->cfg.renorm = renorm ? 1 : 0;
+    drop_graph(s);
+    return NUOSC_OK;
+}
+
+int nuosc_get_dims(const nuosc_sim* s, int* nx, int* nz, int* nv, long long* lpts) {
+    if (!s) return NUOSC_EINVAL;
+    if (nx) *nx = s->nx;
+    if (nz) *nz = s->nz;
+    if (nv) *nv = s->nv;
+    if (lpts) *lpts = (long long)(s->nx + 2 * s->cfg.gx) * (s->nz + 2 * s->cfg.gz) * s->nv;
+    return NUOSC_OK;
+}
+
+int nuosc_get_scalars(const nuosc_sim* s, double* dt, double* ds_L, double* phy_time, double* n_nue0_2) {
+    if (!s) return NUOSC_EINVAL;
+    if (dt) *dt = s->dt;
+    if (ds_L) *ds_L = s->ds_L;
+    if (phy_time) *phy_time = s->phy_time;
+    if (n_nue0_2) {
+        n_nue0_2[0] = s->n_nue0[0];
+        n_nue0_2[1] = s->n_nue0[1];
+    }
+    return NUOSC_OK;
+}
+
+int nuosc_set_phy_time(nuosc_sim* s, double t) {
+    if (!s) return NUOSC_EINVAL;
+    s->phy_time = t;
+    return NUOSC_OK;
+}
+
+int nuosc_get_vgrid(const nuosc_sim* s, double* vx, double* vy, double* vz, double* vw) {
+    if (!s) return NUOSC_EINVAL;
+    const size_t b = sizeof(double) * s->nv;
+    if (vx) std::memcpy(vx, s->vg.vx.data(), b);
+    if (vy) std::memcpy(vy, s->vg.vy.data(), b);
+    if (vz) std::memcpy(vz, s->vg.vz.data(), b);
+    if (vw) std::memcpy(vw, s->vg.vw.data(), b);
+    return NUOSC_OK;
+}
+
+int nuosc_get_coords(const nuosc_sim* s, double* X, double* Z) {
+    if (!s) return NUOSC_EINVAL;
+    if (X) std::memcpy(X, s->X.data(), sizeof(double) * s->nx);
+    if (Z) std::memcpy(Z, s->Z.data(), sizeof(double) * s->nz);
+    return NUOSC_OK;
+}
+
+int nuosc_get_g0(const nuosc_sim* s, double* G0, double* G0b) {
+    if (!s) return NUOSC_EINVAL;
+    if (G0) std::memcpy(G0, s->G0h.data(), sizeof(double) * s->nv);
+    if (G0b) std::memcpy(G0b, s->G0bh.data(), sizeof(double) * s->nv);
+    return NUOSC_OK;
+}
+
+int nuosc_get_device_bytes(const nuosc_sim* s, long long* bytes) {
+    if (!s || !bytes) return NUOSC_EINVAL;
+    *bytes = s->device_bytes;
+    return NUOSC_OK;
+}
+
+int nuosc_fill_init(nuosc_sim* s, int ipt, double alpha, double eps0, double sigma, double lnue, double lnueb,
+                    double lnuex, double lnuebx, uint64_t seed) {
+    if (!s) return NUOSC_EINVAL;
+    if (!(ipt == 0 || ipt == 2 || ipt == 3 || ipt == 5))
+        return fail(s, NUOSC_EINVAL, "ipt must be 0, 2, 3 or 5 (1 and 4 draw from rand() in the reference)");
+    angular_distributions(s->vg, alpha, lnue, lnueb, lnuex, lnuebx, s->G0h, s->G0bh);
+    // per-z perturbation (nuosc_init.cpp:4-6, 98-105)
+    std::vector<double> tab(2 * (size_t)s->nz, 0.0);
+    for (int j = 0; j < s->nz; ++j) {
+        const double z = s->Z[j];
+        double tmpr = 0.0;
+        if (ipt == 0) tmpr = eps0 * std::exp(-(z - 0.0) * (z - 0.0) / (2.0 * sigma * sigma));
+        else if (ipt == 2) tmpr = eps0 * (1.0 + std::cos(2 * M_PI * (z - 0.0) / (2.0 * sigma * sigma))) * 0.5;
+        else if (ipt == 3) tmpr = eps0;
+        tab[j] = tmpr;
+        tab[s->nz + j] = std::sqrt(1.0 - tmpr * tmpr);
+    }
+    std::vector<double> g0(2 * (size_t)s->nv);
+    std::memcpy(g0.data(), s->G0h.data(), sizeof(double) * s->nv);
+    std::memcpy(g0.data() + s->nv, s->G0bh.data(), sizeof(double) * s->nv);
+    HIPCK(s, hipMemcpyAsync(s->d_g0, g0.data(), sizeof(double) * 2 * s->nv, hipMemcpyHostToDevice, s->stream));
+    int rc = ensure_tmp(s, sizeof(double) * 2 * (size_t)s->nz);
+    if (rc) return rc;
+    HIPCK(s, hipMemcpyAsync(s->d_tmp, tab.data(), sizeof(double) * 2 * s->nz, hipMemcpyHostToDevice, s->stream));
+    if (is64(s))
+        HIPCK(s, launch_init<double>(s->geo, ipt, s->d_g0, s->d_g0 + s->nv, s->d_tmp, s->d_tmp + s->nz, eps0, seed,
+                                     (double*)s->st_u, s->stream));
+    else
+        HIPCK(s, launch_init<float>(s->geo, ipt, s->d_g0, s->d_g0 + s->nv, s->d_tmp, s->d_tmp + s->nz, eps0, seed,
+                                    (float*)s->st_u, s->stream));
+    // n_nue0 = ds_L * sum w ee (nuosc_init.cpp:116-130)
+    double h[kNAna];
+    rc = analysis_sums(s, h);
+    if (rc) return rc;
+    s->n_nue0[0] = h[ANA_SURV] * s->ds_L;
+    s->n_nue0[1] = h[ANA_SURVB] * s->ds_L;
+    s->phy_time = 0.0;
+    rc = refresh_moments(s);
+    if (rc) return rc;
+    HIPCK(s, hipStreamSynchronize(s->stream));
+    return NUOSC_OK;
+}
+
+int nuosc_step_rk4(nuosc_sim* s, int nsteps) {
+    if (!s) return NUOSC_EINVAL;
+    if (nsteps < 0) return fail(s, NUOSC_EINVAL, "negative step count");
+    if (!s->have_moments) {
+        int rc = refresh_moments(s);
+        if (rc) return rc;
+    }
+    HIPCK(s, hipEventRecord(s->ev0, s->stream));
+    if (s->cfg.use_graph && nsteps > 0) {
+        if (!s->graph_exec) {
+            HIPCK(s, hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+            int rc = is64(s) ? enqueue_step<double>(s) : enqueue_step<float>(s);
+            hipGraph_t gr = nullptr;
+            hipError_t e = hipStreamEndCapture(s->stream, &gr);
+            if (rc) {
+                if (gr) hipGraphDestroy(gr);
+                return rc;
+            }
+            if (e != hipSuccess) return fail(s, NUOSC_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
+            s->graph = gr;
+            HIPCK(s, hipGraphInstantiate(&s->graph_exec, s->graph, nullptr, nullptr, 0));
+        }
+        for (int n = 0; n < nsteps; ++n) {
+            HIPCK(s, hipGraphLaunch(s->graph_exec, s->stream));
+            s->phy_time += s->dt;
+        }
+    } else {
+        for (int n = 0; n < nsteps; ++n) {
+            int rc = is64(s) ? enqueue_step<double>(s) : enqueue_step<float>(s);
+            if (rc) return rc;
+            s->phy_time += s->dt;  // accumulated, as nuosc_class.cpp:423
+        }
+    }
+    HIPCK(s, hipEventRecord(s->ev1, s->stream));
+    s->steps_in_last_call = nsteps;
+    return NUOSC_OK;
+}
+
+int nuosc_synchronize(nuosc_sim* s) {
+    if (!s) return NUOSC_EINVAL;
+    HIPCK(s, hipStreamSynchronize(s->stream));
+    return NUOSC_OK;
+}
+
+int nuosc_last_step_ms(nuosc_sim* s, double* ms) {
+    if (!s || !ms) return NUOSC_EINVAL;
+    HIPCK(s, hipEventSynchronize(s->ev1));
+    float f = 0;
+    HIPCK(s, hipEventElapsedTime(&f, s->ev0, s->ev1));
+    *ms = f;
+    return NUOSC_OK;
+}
+
+int nuosc_profile_stages(nuosc_sim* s, int nsteps, double* stage_ms) {
+    if (!s || !stage_ms || nsteps < 1) return NUOSC_EINVAL;
+    if (!s->have_moments) {
+        int rc = refresh_moments(s);
+        if (rc) return rc;
+    }
+    hipEvent_t ev[5] = {};
+    for (auto& e : ev)
+        if (hipEventCreate(&e) != hipSuccess) return fail(s, NUOSC_EHIP, "event creation failed");
+    double acc[4] = {0, 0, 0, 0};
+    int rc = NUOSC_OK;
+    for (int n = 0; n < nsteps && rc == NUOSC_OK; ++n) {
+        rc = is64(s) ? enqueue_step<double>(s, ev) : enqueue_step<float>(s, ev);
+        if (rc) break;
+        s->phy_time += s->dt;
+        if (hipEventSynchronize(ev[4]) != hipSuccess) {
+            rc = fail(s, NUOSC_EHIP, "event sync failed");
+            break;
+        }
+        for (int k = 0; k < 4; ++k) {
+            float f = 0;
+            hipEventElapsedTime(&f, ev[k], ev[k + 1]);
+            acc[k] += f;
+        }
+    }
+    for (auto& e : ev) hipEventDestroy(e);
+    for (int k = 0; k < 4; ++k) stage_ms[k] = acc[k] / nsteps;
+    return rc;
+}
+
+int nuosc_calrhs(nuosc_sim* s, const double* in, double* out) {
+    if (!s || !in || !out) return NUOSC_EINVAL;
+    const size_t fbytes = sizeof(double) * (size_t)s->nx * s->nz * s->nv;
+    int rc = ensure_tmp(s, fbytes);
+    if (rc) return rc;
+    const size_t fe = (size_t)s->nx * s->nz * s->nv;
+    for (int f = 0; f < kNVar; ++f) {
+        HIPCK(s, hipMemcpyAsync(s->d_tmp, in + f * fe, fbytes, hipMemcpyHostToDevice, s->stream));
+        if (is64(s)) HIPCK(s, launch_unpack<double>(s->geo, s->d_tmp, field_ptr<double>(s->st_ya, s, f), s->stream));
+        else HIPCK(s, launch_unpack<float>(s->geo, s->d_tmp, field_ptr<float>(s->st_ya, s, f), s->stream));
+    }
+    if (is64(s)) {
+        HIPCK(s, launch_moments<double>(s->geo, (double*)s->st_ya, s->mom_a, s->stream));
+        HIPCK(s, launch_stage<double>(0, s->geo, s->ph, s->plan, (double*)s->st_ya, nullptr, nullptr,
+                                      (double*)s->st_yb, s->mom_a, nullptr, 0.0, 0.0, s->stream));
+    } else {
+        HIPCK(s, launch_moments<float>(s->geo, (float*)s->st_ya, s->mom_a, s->stream));
+        HIPCK(s, launch_stage<float>(0, s->geo, s->ph, s->plan, (float*)s->st_ya, nullptr, nullptr,
+                                     (float*)s->st_yb, s->mom_a, nullptr, 0.0, 0.0, s->stream));
+    }
+    for (int f = 0; f < kNVar; ++f) {
+        if (is64(s)) HIPCK(s, launch_pack<double>(s->geo, field_ptr<double>(s->st_yb, s, f), s->d_tmp, s->stream));
+        else HIPCK(s, launch_pack<float>(s->geo, field_ptr<float>(s->st_yb, s, f), s->d_tmp, s->stream));
+        HIPCK(s, hipMemcpyAsync(out + f * fe, s->d_tmp, fbytes, hipMemcpyDeviceToHost, s->stream));
+    }
+    HIPCK(s, hipStreamSynchronize(s->stream));
+    return NUOSC_OK;
+}
+
+int nuosc_analysis(nuosc_sim* s, nuosc_analysis_t* o) {
+    if (!s || !o) return NUOSC_EINVAL;
+    double h[kNAna];
+    int rc = analysis_sums(s, h);
+    if (rc) return rc;
+    // nuosc_ana.cpp:110-117
+    const double nor = h[ANA_NOR], norb = h[ANA_NORB];
+    o->phy_time = s->phy_time;
+    o->maxdP = h[ANA_MAXDP];
+    o->surv = h[ANA_SURV] / nor;
+    o->survb = h[ANA_SURVB] / norb;
+    o->avgP = h[ANA_AVGP] / nor;
+    o->avgPb = h[ANA_AVGPB] / norb;
+    o->aM0 = std::sqrt(h[ANA_AM01] * h[ANA_AM01] + h[ANA_AM02] * h[ANA_AM02] + h[ANA_AM03] * h[ANA_AM03]) * s->ds_L;
+    o->ELNe = std::fabs(s->n_nue0[0] * (1.0 - o->surv) - s->n_nue0[1] * (1.0 - o->survb)) /
+              (s->n_nue0[0] + s->n_nue0[1]);
+    o->Lex = h[ANA_AM03] * s->ds_L;
+    o->rho_ex2 = h[ANA_REX2] * s->ds_L;
+    o->nonfinite = h[ANA_BAD];
+    // nuosc_ana.cpp:131 asserts maxdP < 10; a NaN state also fails here
+    const bool bad = !(o->maxdP < 10) || h[ANA_BAD] > 0 || !std::isfinite(o->surv) || !std::isfinite(o->aM0);
+    o->status = bad ? NUOSC_EUNSTABLE : NUOSC_OK;
+    if (bad) s->err = "analysis: state unstable (maxdP >= 10 or non-finite values)";
+    return o->status;
+}
+
+int nuosc_renormalize(nuosc_sim* s) {
+    if (!s) return NUOSC_EINVAL;
+    if (is64(s)) HIPCK(s, launch_renorm<double>(s->geo, (double*)s->st_u, s->d_g0, s->d_g0 + s->nv, s->stream));
+    else HIPCK(s, launch_renorm<float>(s->geo, (float*)s->st_u, s->d_g0, s->d_g0 + s->nv, s->stream));
+    return refresh_moments(s);
+}
+
+int nuosc_download_field(nuosc_sim* s, int f, double* host) {
+    if (!s || !host) return NUOSC_EINVAL;
+    int rc = check_field(s, f);
+    if (rc) return rc;
+    const size_t bytes = sizeof(double) * (size_t)s->nx * s->nz * s->nv;
+    rc = ensure_tmp(s, bytes);
+    if (rc) return rc;
+    if (is64(s)) HIPCK(s, launch_pack<double>(s->geo, field_ptr<double>(s->st_u, s, f), s->d_tmp, s->stream));
+    else HIPCK(s, launch_pack<float>(s->geo, field_ptr<float>(s->st_u, s, f), s->d_tmp, s->stream));
+    HIPCK(s, hipMemcpyAsync(host, s->d_tmp, bytes, hipMemcpyDeviceToHost, s->stream));
+    HIPCK(s, hipStreamSynchronize(s->stream));
+    return NUOSC_OK;
+}
+
+int nuosc_download_field_ghosted(nuosc_sim* s, int f, double* host) {
+    if (!s || !host) return NUOSC_EINVAL;
+    int rc = check_field(s, f);
+    if (rc) return rc;
+    const size_t bytes =
+        sizeof(double) * (size_t)(s->nx + 2 * s->cfg.gx) * (s->nz + 2 * s->cfg.gz) * s->nv;
+    rc = ensure_tmp(s, bytes);
+    if (rc) return rc;
+    if (is64(s))
+        HIPCK(s, launch_ghosted<double>(s->geo, s->cfg.gx, s->cfg.gz, field_ptr<double>(s->st_u, s, f), s->d_tmp,
+                                        s->stream));
+    else
+        HIPCK(s, launch_ghosted<float>(s->geo, s->cfg.gx, s->cfg.gz, field_ptr<float>(s->st_u, s, f), s->d_tmp,
+                                       s->stream));
+    HIPCK(s, hipMemcpyAsync(host, s->d_tmp, bytes, hipMemcpyDeviceToHost, s->stream));
+    HIPCK(s, hipStreamSynchronize(s->stream));
+    return NUOSC_OK;
+}
+
+int nuosc_upload_field(nuosc_sim* s, int f, const double* host) {
+    if (!s || !host) return NUOSC_EINVAL;
+    int rc = check_field(s, f);
+    if (rc) return rc;
+    const size_t bytes = sizeof(double) * (size_t)s->nx * s->nz * s->nv;
+    rc = ensure_tmp(s, bytes);
+    if (rc) return rc;
+    HIPCK(s, hipMemcpyAsync(s->d_tmp, host, bytes, hipMemcpyHostToDevice, s->stream));
+    if (is64(s)) HIPCK(s, launch_unpack<double>(s->geo, s->d_tmp, field_ptr<double>(s->st_u, s, f), s->stream));
+    else HIPCK(s, launch_unpack<float>(s->geo, s->d_tmp, field_ptr<float>(s->st_u, s, f), s->stream));
+    HIPCK(s, hipStreamSynchronize(s->stream));
+    s->have_moments = false;
+    return NUOSC_OK;
+}
+
+int nuosc_state_updated(nuosc_sim* s) {
+    if (!s) return NUOSC_EINVAL;
+    int rc = refresh_moments(s);
+    if (rc) return rc;
+    HIPCK(s, hipStreamSynchronize(s->stream));
+    return NUOSC_OK;
+}
+
+int nuosc_p3_slice(nuosc_sim* s, const int* xidx, int sx, const int* vidx, int sv, double* dst) {
+    if (!s || sx < 0 || sv < 0 || (sx && !xidx) || (sv && !vidx) || !dst) return NUOSC_EINVAL;
+    for (int a = 0; a < sx; ++a)
+        if (xidx[a] < 0 || xidx[a] >= s->nx) return fail(s, NUOSC_EINVAL, "x index out of range");
+    for (int b = 0; b < sv; ++b)
+        if (vidx[b] < 0 || vidx[b] >= s->nv) return fail(s, NUOSC_EINVAL, "v index out of range");
+    const size_t n = (size_t)sx * s->nz * sv;
+    if (n == 0) return NUOSC_OK;
+    const size_t ibytes = sizeof(int) * (size_t)(sx + sv);
+    const size_t ibytes_al = (ibytes + 255) / 256 * 256;
+    int rc = ensure_tmp(s, ibytes_al + sizeof(double) * n);
+    if (rc) return rc;
+    int* di = reinterpret_cast<int*>(s->d_tmp);
+    double* dd = reinterpret_cast<double*>(reinterpret_cast<char*>(s->d_tmp) + ibytes_al);
+    std::vector<int> idx(xidx, xidx + sx);
+    idx.insert(idx.end(), vidx, vidx + sv);
+    HIPCK(s, hipMemcpyAsync(di, idx.data(), ibytes, hipMemcpyHostToDevice, s->stream));
+    if (is64(s)) HIPCK(s, launch_p3_slice<double>(s->geo, (double*)s->st_u, s->d_g0, di, sx, di + sx, sv, dd, s->stream));
+    else HIPCK(s, launch_p3_slice<float>(s->geo, (float*)s->st_u, s->d_g0, di, sx, di + sx, sv, dd, s->stream));
+    HIPCK(s, hipMemcpyAsync(dst, dd, sizeof(double) * n, hipMemcpyDeviceToHost, s->stream));
+    HIPCK(s, hipStreamSynchronize(s->stream));
+    return NUOSC_OK;
+}
+
+}  // extern "C"

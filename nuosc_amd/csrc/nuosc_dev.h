@@ -1,0 +1,99 @@
+// nuosc_dev.h -- internal interface between the C-ABI layer (nuosc_capi.cpp)
+// and the HIP kernels (nuosc_kernels.hip).  Not part of the public ABI.
+//
+// Device data layout (one state copy):
+//   8 fields (ee, mm, emr, emi, bee, bmm, bemr, bemi) one after another, each
+//   field [nx][nz][nvp] with v fastest; nvp = nv rounded up so that every site
+//   row starts on a 128-byte boundary.  There is no ghost band: the periodic
+//   boundary is an index wrap inside the stencil loads.
+//   Partial velocity moments: [nx*nz sites][nch chunks][16] doubles, 12 used:
+//   {A0..A3, B0..B3, C0..C3} with A = sum w u (bemr-emr), B = sum w u
+//   (bemi+emi), C = sum w u (bee-bmm-ee+mm), u = (1, vx, vy, vz), each summed
+//   over the 64 directions of one chunk.
+#pragma once
+#include <hip/hip_runtime_api.h>
+#include <cstdint>
+
+namespace nuosc {
+
+constexpr int kNVar = 8;
+constexpr int kWave = 64;
+constexpr int kMomStride = 16;   // doubles per (site, chunk) moment record
+constexpr int kNMom = 12;
+constexpr int kNAna = 12;        // analysis accumulators (see AnaIdx)
+
+enum AnaIdx {
+    ANA_SURV = 0, ANA_SURVB, ANA_AVGP, ANA_AVGPB, ANA_AM01, ANA_AM02, ANA_AM03,
+    ANA_NOR, ANA_NORB, ANA_REX2, ANA_MAXDP, ANA_BAD
+};
+
+struct Geometry {
+    int nx, nz, nv, nvp, nch;
+    long long fs;          // elements per field
+    double twelve_dx, twelve_dz;   // 12*dx, 12*dz (advection denominators)
+    double ko_x, ko_z;     // -ko/dx/16, -ko/dz/16
+    const double *vx, *vy, *vz, *vw;   // device, length nv
+    int bc;                // 0 periodic
+};
+
+struct Physics {
+    double mu, pmo, ct, st;
+};
+
+// One RK4 stage over the whole grid.
+//   stage 0: yout = L(yin)                       (RHS only; test primitive)
+//   stage 1: w = yin + cw*k,  yout = yin + cy*k   (yin is the state u)
+//   stage 2/3: w = w + cw*k,  yout = u + cy*k
+//   stage 4: yout = w + cw*k                      (yout is the new state)
+// min holds the partial moments of yin; mout (stages 1-4) receives those of yout.
+struct StagePlan {
+    int zlen, nzs, ntask, nblocks;
+};
+
+StagePlan plan_stage(const Geometry& g, int num_cus);
+
+template <typename T>
+hipError_t launch_stage(int stage, const Geometry& g, const Physics& ph, const StagePlan& pl,
+                        const T* yin, const T* u, T* w, T* yout, const double* min, double* mout,
+                        double cw, double cy, hipStream_t s);
+
+template <typename T>
+hipError_t launch_moments(const Geometry& g, const T* y, double* mout, hipStream_t s);
+
+// Initial data.  ipt 0/2/3: per-z perturbation tables (tmpr, p3o) computed on
+// the host; ipt 5: seeded complex perturbation generated per point.
+template <typename T>
+hipError_t launch_init(const Geometry& g, int ipt, const double* G0, const double* G0b,
+                       const double* tmpr_z, const double* p3o_z, double eps0, uint64_t seed,
+                       T* y, hipStream_t s);
+
+// Diagnostics: per-block partials then one fixed-order final reduction into
+// out[kNAna] (device).  part must hold ana_blocks() * kNAna * 2 doubles.
+int ana_blocks(const Geometry& g);
+template <typename T>
+hipError_t launch_analysis(const Geometry& g, const T* y, const double* G0, const double* G0b,
+                           double* part, double* out, hipStream_t s);
+
+// Reference-layout copy of one field with its ghost band filled by the
+// periodic rule (corners zero), as FP64: dst[(nx+2gx)*(nz+2gz)*nv].
+template <typename T>
+hipError_t launch_ghosted(const Geometry& g, int gx, int gz, const T* field, double* dst,
+                          hipStream_t s);
+
+// Interior field <-> FP64 dense [nx][nz][nv].
+template <typename T>
+hipError_t launch_pack(const Geometry& g, const T* field, double* dst, hipStream_t s);
+template <typename T>
+hipError_t launch_unpack(const Geometry& g, const double* src, T* field, hipStream_t s);
+
+// P3 = (ee-mm)/G0 at x indices xi[sx] and v indices vi[sv]: dst[sx][nz][sv].
+template <typename T>
+hipError_t launch_p3_slice(const Geometry& g, const T* y, const double* G0, const int* xi, int sx,
+                           const int* vi, int sv, double* dst, hipStream_t s);
+
+// Renormalisation of the state to |P| = 1 (the reference's formula).
+template <typename T>
+hipError_t launch_renorm(const Geometry& g, T* y, const double* G0, const double* G0b,
+                         hipStream_t s);
+
+}  // namespace nuosc

@@ -1,0 +1,114 @@
+/*
+ * nuosc_oracle.h -- CPU restatement of the nuosc RK4 time-stepper core.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity checker for the HIP
+ * product path (libnuosc_hip.so).  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it.  The product never links it.
+ *
+ * It restates, in plain C99 + OpenMP, the reference's algorithm with the
+ * reference's own storage layout (8 SoA fields with a ghost band, index
+ * ((i+gx)*(nz+2gz)+j+gz)*nv+v, CartGrid.h:91) and the reference's operation
+ * order, so that its results reproduce the reference's arithmetic:
+ *
+ *   orc_vgrid_icosahedron   icosahedron.h:36-171, Point.h:64-69, CartGrid.cpp:46-61
+ *   orc_create (geometry)   CartGrid.cpp:191-208, nuosc_class.cpp:454-466
+ *   orc_fill_init           nuosc_init.cpp:4-137 (ipt 0/2/3) + seeded ipt 5
+ *   orc_periodic_ghosts     nuosc_boundary.cpp:4-52
+ *   orc_calrhs              nuosc_class.cpp:11-307 (2 flavours, KO_ORD_3,
+ *                           central 4th-order advection, vacuum on)
+ *   orc_step_rk4            nuosc_class.cpp:312-351, 374-427
+ *   orc_analysis            nuosc_ana.cpp:3-137
+ *   orc_renormalize         nuosc_ana.cpp:140-166
+ *
+ * How each function is pinned (tests/test_oracle_*.py):
+ *   - v-grid: bit-exact against the reference's own icosahedron.h compiled
+ *     where it lies (oracle/_ref/ref_vgrid) and the committed fixtures made by
+ *     it; closed forms N=10nn^2+2, |v|=1, sum w v = 0.
+ *   - ghosts: brute-force modular-index check.
+ *   - calRHS: an independent numpy formulation (2x2 complex density matrices,
+ *     H = mu*sum_k w_k(1-v.v_k)(rho_k - conj(rhobar_k)) + pmo(st s1 - ct s3),
+ *     rhs = -i[H,rho], np.roll finite differences); single-Fourier-mode symbol
+ *     of ADV+KO.
+ *   - RK4: exact amplification G(z)^n of a Fourier mode; vacuum precession
+ *     polynomial; conservation of sum_xz(ee+mm) per direction.
+ *   - init: sum_v w G0 = 1, sum_v w G0b = alpha, trace and purity identities.
+ *   - analysis: an independent numpy formulation of the diagnostics plus the
+ *     pure-state/conservation identities.
+ * Nothing else of the reference builds here: every other reference source
+ * includes common.h, which hard-wires COSENU_MPI and therefore <mpi.h>,
+ * which this image does not have.
+ */
+#ifndef NUOSC_ORACLE_H
+#define NUOSC_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_NVAR 8
+
+typedef struct orc_sim {
+    /* geometry (CartGrid.cpp:191-208) */
+    int nx, nz, nv, gx, gz;
+    double x0, x1, z0, z1, dx, dz;
+    double *X, *Z;
+    double *vx, *vy, *vz, *vw;
+    long lpts;
+    /* physics (nuosc_class.h:99-126, nuosc_class.cpp:454-466) */
+    double dt, ds_L, cfl, ko, mu, pmo, ct, st;
+    int renorm;
+    double phy_time;
+    double n_nue0[2];
+    double *G0, *G0b;          /* per direction (the reference stores them per point) */
+    /* four RK4 state buffers, 8 fields each (nuosc_class.h:106-156) */
+    double *stat[ORC_NVAR], *rhs[ORC_NVAR], *pre[ORC_NVAR], *cor[ORC_NVAR];
+    int analysis_lines;        /* analysis.dat line counter (first line precision quirk) */
+} orc_sim;
+
+/* Velocity grid.  Returns N = 10*nn*nn+2 and fills the four arrays (length N). */
+int orc_vgrid_icosahedron(int nn, double* vx, double* vy, double* vz, double* vw);
+
+/* Create.  vgrid_mode 0 = icosahedral Voronoi weights (reference), 1 = equal
+ * weights 4*pi/N on the same directions.  Returns NULL on bad input. */
+orc_sim* orc_create(int nn, int gx, int gz, double x0, double x1, double z0, double z1,
+                    double dx, double dz, double cfl, double ko, int vgrid_mode);
+void orc_destroy(orc_sim* s);
+void orc_set_physics(orc_sim* s, double mu, double pmo, int renorm);
+
+/* nuosc_init.cpp:17-137.  ipt 0,2,3 as the reference; ipt 5 = seeded complex
+ * perturbation (counter-based RNG keyed by seed and the interior index).
+ * Returns 0 on success, -1 for an unsupported ipt. */
+int orc_fill_init(orc_sim* s, int ipt, double alpha, double eps0, double sigma,
+                  double lnue, double lnueb, double lnuex, double lnuebx, uint64_t seed);
+
+/* The two uniforms in [-1,1) used by ipt 5 for interior point (i,j,v). */
+void orc_seed_uniforms(uint64_t seed, long lin, double* ur, double* ui);
+
+long orc_idx(const orc_sim* s, int i, int j, int v);
+
+void orc_periodic_ghosts(orc_sim* s, double* const f[ORC_NVAR]);
+void orc_calrhs(const orc_sim* s, double* const out[ORC_NVAR], double* const in[ORC_NVAR]);
+void orc_step_rk4(orc_sim* s);
+void orc_renormalize(orc_sim* s, double* const f[ORC_NVAR]);
+
+/* Diagnostics (nuosc_ana.cpp:34-137).  out[9] = {phy_time, maxdP, surv, survb,
+ * avgP, avgPb, aM0, Lex, ELNe}.  Returns 0, or 1 when maxdP >= 10 (the
+ * reference's assert) or a sum is not finite. */
+int orc_analysis(orc_sim* s, double out[9]);
+/* Write the analysis.dat line for out[] with the reference's stream formatting;
+ * returns the number of characters written to buf (buf >= 512 bytes). */
+int orc_format_analysis_line(orc_sim* s, const double out[9], char* buf, int buflen);
+
+/* Field access in the interior layout [nx][nz][nv] (f in 0..7 with the
+ * reference's field order ee, mm, emr, emi, bee, bmm, bemr, bemi). */
+void orc_get_field(const orc_sim* s, int which_buf, int f, double* dst);
+void orc_set_field(orc_sim* s, int which_buf, int f, const double* src);
+/* Whole ghost-inclusive buffer (lpts doubles). */
+double* orc_field_ptr(orc_sim* s, int which_buf, int f);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
