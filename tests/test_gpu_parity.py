@@ -51,8 +51,10 @@ def test_setup_bit_exact(oracle_mod, nuosc_mod):
         assert np.array_equal(G0, o.G0) and np.array_equal(G0b, o.G0b)
         gs, os_ = g.get_state(), o.get_state()
         assert np.array_equal(gs, os_), f"ipt={ipt}: initial state differs"
-        # n_nue0 is a sum over all points: only the summation order differs
-        assert np.allclose(g.n_nue0, o.n_nue0, rtol=1e-14, atol=0)
+        # n_nue0 sums 663552 terms: the oracle (like the reference) adds them
+        # sequentially, which alone carries ~1e-12 relative error; the library
+        # uses compensated sums
+        assert np.allclose(g.n_nue0, o.n_nue0, rtol=1e-11, atol=0)
 
 
 def test_seeded_init(oracle_mod, nuosc_mod):
