@@ -38,7 +38,8 @@ struct nuosc_sim {
     void* st_w = nullptr;
     void* st_ya = nullptr;
     void* st_yb = nullptr;
-    double* mom_u = nullptr;
+    double* mom_part = nullptr;  // partial records of the last stage output
+    double* mom_u = nullptr;     // totals of u, yA, yB
     double* mom_a = nullptr;
     double* mom_b = nullptr;
     double* d_vgrid = nullptr;  // vx | vy | vz | vw
@@ -115,25 +116,34 @@ int enqueue_step(nuosc_sim* s, hipEvent_t* ev = nullptr) {
     T* ya = static_cast<T*>(s->st_ya);
     T* yb = static_cast<T*>(s->st_yb);
     const Geometry& g = s->geo;
+    const bool comp = sizeof(T) == 4;
+    double* P = s->mom_part;
     if (ev) HIPCK(s, hipEventRecord(ev[0], s->stream));
-    HIPCK(s, launch_stage<T>(1, g, s->ph, s->plan, u, nullptr, w, ya, s->mom_u, s->mom_a, c6, c2, s->stream));
+    HIPCK(s, launch_stage<T>(1, g, s->ph, s->plan, u, nullptr, w, ya, s->mom_u, P, c6, c2, s->stream));
+    HIPCK(s, launch_moment_totals(g, comp, P, s->mom_a, s->stream));
     if (ev) HIPCK(s, hipEventRecord(ev[1], s->stream));
-    HIPCK(s, launch_stage<T>(2, g, s->ph, s->plan, ya, u, w, yb, s->mom_a, s->mom_b, c3, c2, s->stream));
+    HIPCK(s, launch_stage<T>(2, g, s->ph, s->plan, ya, u, w, yb, s->mom_a, P, c3, c2, s->stream));
+    HIPCK(s, launch_moment_totals(g, comp, P, s->mom_b, s->stream));
     if (ev) HIPCK(s, hipEventRecord(ev[2], s->stream));
-    HIPCK(s, launch_stage<T>(3, g, s->ph, s->plan, yb, u, w, ya, s->mom_b, s->mom_a, c3, dt, s->stream));
+    HIPCK(s, launch_stage<T>(3, g, s->ph, s->plan, yb, u, w, ya, s->mom_b, P, c3, dt, s->stream));
+    HIPCK(s, launch_moment_totals(g, comp, P, s->mom_a, s->stream));
     if (ev) HIPCK(s, hipEventRecord(ev[3], s->stream));
-    HIPCK(s, launch_stage<T>(4, g, s->ph, s->plan, ya, nullptr, w, u, s->mom_a, s->mom_u, c6, 0.0, s->stream));
+    HIPCK(s, launch_stage<T>(4, g, s->ph, s->plan, ya, nullptr, w, u, s->mom_a, P, c6, 0.0, s->stream));
+    HIPCK(s, launch_moment_totals(g, comp, P, s->mom_u, s->stream));
     if (ev) HIPCK(s, hipEventRecord(ev[4], s->stream));
     if (s->cfg.renorm) {
         HIPCK(s, launch_renorm<T>(g, u, s->d_g0, s->d_g0 + s->nv, s->stream));
-        HIPCK(s, launch_moments<T>(g, u, s->mom_u, s->stream));
+        HIPCK(s, launch_moments<T>(g, u, P, s->stream));
+        HIPCK(s, launch_moment_totals(g, comp, P, s->mom_u, s->stream));
     }
     return NUOSC_OK;
 }
 
+// partial records and totals of the state u
 int refresh_moments(nuosc_sim* s) {
-    if (is64(s)) HIPCK(s, launch_moments<double>(s->geo, (double*)s->st_u, s->mom_u, s->stream));
-    else HIPCK(s, launch_moments<float>(s->geo, (float*)s->st_u, s->mom_u, s->stream));
+    if (is64(s)) HIPCK(s, launch_moments<double>(s->geo, (double*)s->st_u, s->mom_part, s->stream));
+    else HIPCK(s, launch_moments<float>(s->geo, (float*)s->st_u, s->mom_part, s->stream));
+    HIPCK(s, launch_moment_totals(s->geo, !is64(s), s->mom_part, s->mom_u, s->stream));
     s->have_moments = true;
     return NUOSC_OK;
 }
@@ -218,6 +228,10 @@ int nuosc_create(const nuosc_config* cfg, nuosc_sim** out) {
     s->nvp = (s->nv + align - 1) / align * align;
     s->nch = (s->nv + kWave - 1) / kWave;
     s->fs = (long long)s->nx * s->nz * s->nvp;
+    if (s->fs * (long long)s->esize >= (1ll << 32) - 1) {
+        delete s;
+        return fail(nullptr, NUOSC_EINVAL, "one field must stay below 4 GiB (nx*nz*nv too large for one GPU handle)");
+    }
     // nuosc_class.cpp:461-464, nuosc_class.h:120-122
     s->ds_L = c.dx * c.dz / (c.z1 - c.z0) / (c.x1 - c.x0);
     s->dt = c.dz * c.cfl;
@@ -238,18 +252,20 @@ int nuosc_create(const nuosc_config* cfg, nuosc_sim** out) {
         return bail(NUOSC_EHIP, "event creation failed");
 
     const size_t state_bytes = (size_t)kNVar * s->fs * s->esize;
-    const size_t mom_bytes = (size_t)s->nx * s->nz * s->nch * kMomStride * sizeof(double);
+    const size_t part_bytes = (size_t)s->nx * s->nz * s->nch * kMomStride * sizeof(double);
+    const size_t mom_bytes = (size_t)s->nx * s->nz * kMomStride * sizeof(double);
     void** states[4] = {&s->st_u, &s->st_w, &s->st_ya, &s->st_yb};
     for (void** p : states) {
         if (hipMalloc(p, state_bytes) != hipSuccess) return bail(NUOSC_ENOMEM, "state allocation failed");
         s->device_bytes += (long long)state_bytes;
         if (hipMemsetAsync(*p, 0, state_bytes, s->stream) != hipSuccess) return bail(NUOSC_EHIP, "memset failed");
     }
-    double** moms[3] = {&s->mom_u, &s->mom_a, &s->mom_b};
+    double** moms[4] = {&s->mom_part, &s->mom_u, &s->mom_a, &s->mom_b};
     for (double** p : moms) {
-        if (hipMalloc((void**)p, mom_bytes) != hipSuccess) return bail(NUOSC_ENOMEM, "moment allocation failed");
-        s->device_bytes += (long long)mom_bytes;
-        if (hipMemsetAsync(*p, 0, mom_bytes, s->stream) != hipSuccess) return bail(NUOSC_EHIP, "memset failed");
+        const size_t b = (p == &s->mom_part) ? part_bytes : mom_bytes;
+        if (hipMalloc((void**)p, b) != hipSuccess) return bail(NUOSC_ENOMEM, "moment allocation failed");
+        s->device_bytes += (long long)b;
+        if (hipMemsetAsync(*p, 0, b, s->stream) != hipSuccess) return bail(NUOSC_EHIP, "memset failed");
     }
     const size_t vbytes = 4 * (size_t)s->nv * sizeof(double);
     if (hipMalloc((void**)&s->d_vgrid, vbytes) != hipSuccess) return bail(NUOSC_ENOMEM, "allocation failed");
@@ -297,7 +313,7 @@ int nuosc_destroy(nuosc_sim* s) {
     if (!s) return NUOSC_OK;
     if (s->stream) hipStreamSynchronize(s->stream);
     drop_graph(s);
-    void* bufs[] = {s->st_u, s->st_w, s->st_ya, s->st_yb, s->mom_u, s->mom_a, s->mom_b,
+    void* bufs[] = {s->st_u, s->st_w, s->st_ya, s->st_yb, s->mom_part, s->mom_u, s->mom_a, s->mom_b,
                     s->d_vgrid, s->d_g0, s->d_part, s->d_ana, s->d_tmp};
     for (void* p : bufs)
         if (p) hipFree(p);
@@ -526,11 +542,13 @@ int nuosc_calrhs(nuosc_sim* s, const double* in, double* out) {
         else HIPCK(s, launch_unpack<float>(s->geo, s->d_tmp, field_ptr<float>(s->st_ya, s, f), s->stream));
     }
     if (is64(s)) {
-        HIPCK(s, launch_moments<double>(s->geo, (double*)s->st_ya, s->mom_a, s->stream));
+        HIPCK(s, launch_moments<double>(s->geo, (double*)s->st_ya, s->mom_part, s->stream));
+        HIPCK(s, launch_moment_totals(s->geo, false, s->mom_part, s->mom_a, s->stream));
         HIPCK(s, launch_stage<double>(0, s->geo, s->ph, s->plan, (double*)s->st_ya, nullptr, nullptr,
                                       (double*)s->st_yb, s->mom_a, nullptr, 0.0, 0.0, s->stream));
     } else {
-        HIPCK(s, launch_moments<float>(s->geo, (float*)s->st_ya, s->mom_a, s->stream));
+        HIPCK(s, launch_moments<float>(s->geo, (float*)s->st_ya, s->mom_part, s->stream));
+        HIPCK(s, launch_moment_totals(s->geo, true, s->mom_part, s->mom_a, s->stream));
         HIPCK(s, launch_stage<float>(0, s->geo, s->ph, s->plan, (float*)s->st_ya, nullptr, nullptr,
                                      (float*)s->st_yb, s->mom_a, nullptr, 0.0, 0.0, s->stream));
     }

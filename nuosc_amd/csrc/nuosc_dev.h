@@ -6,10 +6,12 @@
 //   field [nx][nz][nvp] with v fastest; nvp = nv rounded up so that every site
 //   row starts on a 128-byte boundary.  There is no ghost band: the periodic
 //   boundary is an index wrap inside the stencil loads.
-//   Partial velocity moments: [nx*nz sites][nch chunks][16] doubles, 12 used:
-//   {A0..A3, B0..B3, C0..C3} with A = sum w u (bemr-emr), B = sum w u
-//   (bemi+emi), C = sum w u (bee-bmm-ee+mm), u = (1, vx, vy, vz), each summed
-//   over the 64 directions of one chunk.
+//   Velocity moments: 12 doubles {A0..A3, B0..B3, C0..C3} per site with
+//   A = sum w u (bemr-emr), B = sum w u (bemi+emi), C = sum w u
+//   (bee-bmm-ee+mm), u = (1, vx, vy, vz).  The stage kernels write partial
+//   records [nx*nz sites][nch chunks][16] (each summed over the 64 directions
+//   of one chunk); k_moment_totals adds them in chunk order into totals
+//   [nx*nz sites][16], which the next stage reads.
 #pragma once
 #include <hip/hip_runtime_api.h>
 #include <cstdint>
@@ -45,20 +47,29 @@ struct Physics {
 //   stage 1: w = yin + cw*k,  yout = yin + cy*k   (yin is the state u)
 //   stage 2/3: w = w + cw*k,  yout = u + cy*k
 //   stage 4: yout = w + cw*k                      (yout is the new state)
-// min holds the partial moments of yin; mout (stages 1-4) receives those of yout.
+// mtot holds the moment totals of yin; mpart (stages 1-4) receives the partial
+// moments of yout.
 struct StagePlan {
     int zlen, nzs, ntask, nblocks;
 };
 
 StagePlan plan_stage(const Geometry& g, int num_cus);
+// The stage kernel addresses each field through a buffer descriptor: one
+// field must span fewer than 4 GiB.
+bool buffer_addressable(const Geometry& g, int esize);
 
 template <typename T>
 hipError_t launch_stage(int stage, const Geometry& g, const Physics& ph, const StagePlan& pl,
-                        const T* yin, const T* u, T* w, T* yout, const double* min, double* mout,
+                        const T* yin, const T* u, T* w, T* yout, const double* mtot, double* mpart,
                         double cw, double cy, hipStream_t s);
 
+// partial records -> per-site totals (compensated sums for the FP32 form)
+hipError_t launch_moment_totals(const Geometry& g, bool compensated, const double* part, double* tot,
+                                hipStream_t s);
+
+// partial moment records of a state (after init / upload)
 template <typename T>
-hipError_t launch_moments(const Geometry& g, const T* y, double* mout, hipStream_t s);
+hipError_t launch_moments(const Geometry& g, const T* y, double* mpart, hipStream_t s);
 
 // Initial data.  ipt 0/2/3: per-z perturbation tables (tmpr, p3o) computed on
 // the host; ipt 5: seeded complex perturbation generated per point.

@@ -146,61 +146,94 @@ __device__ __forceinline__ void moment_terms(const T (&y)[kNVar], const double (
     for (int s = 12; s < 16; ++s) t[s] = 0.0;
 }
 
-// Sum of the nch partial moment records of one site, in chunk order.
-template <bool COMP>
-__device__ __forceinline__ void site_moments(const double* __restrict__ rec, int nch,
-                                             double (&M)[kNMom]) {
-    if constexpr (!COMP) {
-#pragma unroll
-        for (int s = 0; s < kNMom; ++s) M[s] = rec[s];
-        for (int c = 1; c < nch; ++c) {
-#pragma unroll
-            for (int s = 0; s < kNMom; ++s) M[s] += rec[c * kMomStride + s];
-        }
-    } else {
-        double E[kNMom];
-#pragma unroll
-        for (int s = 0; s < kNMom; ++s) { M[s] = rec[s]; E[s] = 0.0; }
-        for (int c = 1; c < nch; ++c) {
-#pragma unroll
-            for (int s = 0; s < kNMom; ++s) {
-                double sum, err;
-                two_sum(M[s], rec[c * kMomStride + s], sum, err);
-                M[s] = sum;
-                E[s] += err;
-            }
-        }
-#pragma unroll
-        for (int s = 0; s < kNMom; ++s) M[s] += E[s];
+// ---------------------------------------------------------------------------
+// buffer addressing: one buffer resource per field (SGPR base), the uniform
+// site offset in soffset and the lane's direction offset in voffset, so no
+// per-load address arithmetic lives in VGPRs
+// ---------------------------------------------------------------------------
+typedef __attribute__((ext_vector_type(2))) unsigned int u32x2;
+constexpr int kRsrcWord3 = 0x00020000;  // gfx9 raw-buffer descriptor word 3
+
+template <typename T>
+struct Lane;
+template <>
+struct Lane<double> {
+    __device__ static __forceinline__ double ld(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so) {
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, 0));
     }
-}
+    __device__ static __forceinline__ void st(double x, __amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so) {
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, x), r, vo, so, 0);
+    }
+};
+template <>
+struct Lane<float> {
+    __device__ static __forceinline__ float ld(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so) {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, 0));
+    }
+    __device__ static __forceinline__ void st(float x, __amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so) {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x), r, vo, so, 0);
+    }
+};
+
+// The 8 fields of one state copy; field f starts at base + f*fs.  Groups of
+// FPD consecutive fields share one buffer descriptor (a group spans fewer
+// than 4 GiB); the field's offset inside its group goes into the scalar
+// offset together with the site offset.  Descriptors are rebuilt from the
+// 64-bit base at each use (a few scalar instructions): kept live across the
+// z loop, the descriptors of four state copies overflow the scalar register
+// file.
+template <typename T, int FPD>
+struct Fields {
+    unsigned long long base;   // address of field 0
+    unsigned long long group_stride;  // FPD * field bytes
+    unsigned group_bytes;
+    unsigned field_bytes;
+    // base address with an empty asm barrier: the compiler cannot hoist the
+    // descriptors built from it out of the loop
+    __device__ __forceinline__ unsigned long long fresh_base() const {
+        unsigned long long b = base;
+        asm volatile("" : "+s"(b));
+        return b;
+    }
+    __device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(unsigned long long b, int f) const {
+        return __builtin_amdgcn_make_buffer_rsrc((void*)(b + (unsigned long long)(f / FPD) * group_stride), 0,
+                                                 (int)group_bytes, kRsrcWord3);
+    }
+    __device__ __forceinline__ unsigned foff(int f) const { return (unsigned)(f % FPD) * field_bytes; }
+    __device__ __forceinline__ T ld(unsigned long long b, int f, unsigned vo, unsigned so) const {
+        return Lane<T>::ld(rsrc(b, f), vo, so + foff(f));
+    }
+    __device__ __forceinline__ void st(unsigned long long b, int f, T x, unsigned vo, unsigned so) const {
+        Lane<T>::st(x, rsrc(b, f), vo, so + foff(f));
+    }
+};
+
+// constant address space: uniform loads from it become scalar loads
+typedef const __attribute__((address_space(4))) double cdouble;
 
 // ---------------------------------------------------------------------------
 // RK4 stage kernel
 // ---------------------------------------------------------------------------
-template <typename T>
+template <typename T, int FPD>
 struct StageArgs {
     Geometry g;
-    const T* __restrict__ yin;
-    const T* __restrict__ u;
-    T* __restrict__ w;
-    T* __restrict__ yout;
-    const double* __restrict__ min;
-    double* __restrict__ mout;
+    Fields<T, FPD> yin, u, w, yout;
+    const double* mtot;     // moment totals of yin: [site][kMomStride]
+    double* mpart;          // partial moments of yout: [site][nch][kMomStride]
     double cw, cy;
     double mu2, mu, p2st, p2ct, pmo, ct2, st;
     int zlen, ntask;
 };
 
-template <typename T, int STAGE, bool COMP>
-__global__ __launch_bounds__(256) void k_stage(const StageArgs<T> a) {
+template <typename T, int FPD, int STAGE, bool COMP>
+__global__ __launch_bounds__(256) void k_stage(const StageArgs<T, FPD> a) {
     const int lane = threadIdx.x & 63;
     const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int task = xcd_remap(blockIdx.x, gridDim.x) * 4 + wib;
     if (task >= a.ntask) return;
 
-    const int nx = a.g.nx, nz = a.g.nz, nvp = a.g.nvp, nch = a.g.nch, nv = a.g.nv;
-    const long long fs = a.g.fs;
+    const int nx = a.g.nx, nz = a.g.nz, nch = a.g.nch, nv = a.g.nv;
+    const unsigned rowb = (unsigned)a.g.nvp * sizeof(T);   // bytes per site row
     const int i = task % nx;
     const int rest = task / nx;
     const int ch = rest % nch;
@@ -208,6 +241,7 @@ __global__ __launch_bounds__(256) void k_stage(const StageArgs<T> a) {
     const int v = ch * kWave + lane;
     const bool act = v < nv;
     const int vl = act ? v : nv - 1;
+    const unsigned vo = (unsigned)vl * sizeof(T);
 
     // per-direction constants (nuosc_class.cpp:119-121, 180-181)
     const double vxd = a.g.vx[vl], vyd = a.g.vy[vl], vzd = a.g.vz[vl];
@@ -220,18 +254,23 @@ __global__ __launch_bounds__(256) void k_stage(const StageArgs<T> a) {
     const T pmo = (T)a.pmo, ct2 = (T)a.ct2, st = (T)a.st;
     const T cw = (T)a.cw, cy = (T)a.cy;
 
-    // row bases (site index of (i', 0))
-    const long long r0 = (long long)i * nz;
-    const long long rm1 = (long long)wrapi(i - 1, nx) * nz;
-    const long long rm2 = (long long)wrapi(i - 2, nx) * nz;
-    const long long rp1 = (long long)wrapi(i + 1, nx) * nz;
-    const long long rp2 = (long long)wrapi(i + 2, nx) * nz;
+    // The five x rows i-2..i+2 of the stencil (periodic wrap).  Loads use the
+    // lowest of them as the scalar base and a per-row byte delta (>= 0) in
+    // the vector offset, so one scalar offset per field group serves all
+    // five rows.
+    int rows[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) rows[k] = wrapi(i - 2 + k, nx);
+    int rmin = rows[0];
+#pragma unroll
+    for (int k = 1; k < 5; ++k) rmin = min(rmin, rows[k]);
+    unsigned vrow[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) vrow[k] = (unsigned)(rows[k] - rmin) * (unsigned)nz * rowb + vo;
+    const unsigned r0 = (unsigned)i * nz;
+    const unsigned rbase = (unsigned)rmin * nz;
 
-    const T* __restrict__ yin = a.yin;
-    auto ld = [&](const T* __restrict__ base, int f, long long site) -> T {
-        return base[f * fs + site * nvp + vl];
-    };
-
+    cdouble* mtot = (cdouble*)a.mtot;
     const int j0 = zs * a.zlen;
     const int j1 = min(nz, j0 + a.zlen);
 
@@ -239,25 +278,54 @@ __global__ __launch_bounds__(256) void k_stage(const StageArgs<T> a) {
     T win[kNVar][5];
 #pragma unroll
     for (int qz = 0; qz < 4; ++qz) {
-        const long long site = r0 + wrapi(j0 - 2 + qz, nz);
+        const unsigned so = (rbase + (unsigned)wrapi(j0 - 2 + qz, nz)) * rowb;
 #pragma unroll
-        for (int f = 0; f < kNVar; ++f) win[f][qz] = ld(yin, f, site);
+        for (int f = 0; f < kNVar; ++f) win[f][qz] = a.yin.ld(a.yin.base, f, vrow[2], so);
     }
+    // moments of the first site; later ones are fetched one site ahead
+    double M[kNMom];
+#pragma unroll
+    for (int s = 0; s < kNMom; ++s) M[s] = mtot[(long long)(r0 + j0) * kMomStride + s];
 
     for (int j = j0; j < j1; ++j) {
-        const long long site = r0 + j;
+        const unsigned site = r0 + j;
+        const unsigned so = site * rowb;        // own row, own offsets
+        const unsigned sx = (rbase + j) * rowb;  // x rows, with vrow[k]
+        // all loads of this iteration first
+        const unsigned long long by = a.yin.fresh_base();
         {
-            const long long s2 = r0 + wrapi(j + 2, nz);
+            const unsigned s2 = (rbase + (unsigned)wrapi(j + 2, nz)) * rowb;
 #pragma unroll
-            for (int f = 0; f < kNVar; ++f) win[f][4] = ld(yin, f, s2);
+            for (int f = 0; f < kNVar; ++f) win[f][4] = a.yin.ld(by, f, vrow[2], s2);
         }
-        // velocity moments of the stage input at this site
-        double M[kNMom];
-        site_moments<COMP>(a.min + site * nch * kMomStride, nch, M);
-        // projections onto direction v (INP, nuosc_class.cpp:187)
+        T xm2[kNVar], xm1[kNVar], xp1[kNVar], xp2[kNVar];
+#pragma unroll
+        for (int f = 0; f < kNVar; ++f) {
+            xm2[f] = a.yin.ld(by, f, vrow[0], sx);
+            xm1[f] = a.yin.ld(by, f, vrow[1], sx);
+            xp1[f] = a.yin.ld(by, f, vrow[3], sx);
+            xp2[f] = a.yin.ld(by, f, vrow[4], sx);
+        }
+        T uc[kNVar], wc[kNVar];
+        if constexpr (STAGE == 2 || STAGE == 3) {
+            const unsigned long long bu = a.u.fresh_base();
+#pragma unroll
+            for (int f = 0; f < kNVar; ++f) uc[f] = a.u.ld(bu, f, vo, so);
+        }
+        if constexpr (STAGE >= 2) {
+            const unsigned long long bw = a.w.fresh_base();
+#pragma unroll
+            for (int f = 0; f < kNVar; ++f) wc[f] = a.w.ld(bw, f, vo, so);
+        }
+        // projections of the site's velocity moments onto direction v
+        // (INP, nuosc_class.cpp:187)
         const T A = (T)(M[0] - M[1] * vxd - M[2] * vyd - M[3] * vzd);
         const T B = (T)(M[4] - M[5] * vxd - M[6] * vyd - M[7] * vzd);
         const T C = (T)(M[8] - M[9] * vxd - M[10] * vyd - M[11] * vzd);
+        if (j + 1 < j1) {
+#pragma unroll
+            for (int s = 0; s < kNMom; ++s) M[s] = mtot[(long long)(site + 1) * kMomStride + s];
+        }
 
         const T ee = win[0][2], mm = win[1][2], emr = win[2][2], emi = win[3][2];
         const T bee = win[4][2], bmm = win[5][2], bemr = win[6][2], bemi = win[7][2];
@@ -269,77 +337,52 @@ __global__ __launch_bounds__(256) void k_stage(const StageArgs<T> a) {
         const T Ibemr = -mu * ((bee - bmm) * B + C * bemi);
         const T Ibemi = mu * ((bmm - bee) * A + C * bemr);
         // local terms (interaction + vacuum, nuosc_class.cpp:280-287)
-        T loc[kNVar];
-        loc[0] = Iee;
-        loc[1] = -Iee;
-        loc[2] = Iemr;
-        loc[3] = Iemi;
-        loc[4] = Ibee;
-        loc[5] = -Ibee;
-        loc[6] = Ibemr;
-        loc[7] = Ibemi;
-        T vac[kNVar];
-        vac[0] = -(p2st * emi);
-        vac[1] = p2st * emi;
-        vac[2] = -(p2ct * emi);
-        vac[3] = pmo * (ct2 * emr + st * (ee - mm));
-        vac[4] = -(p2st * bemi);
-        vac[5] = p2st * bemi;
-        vac[6] = -(p2ct * bemi);
-        vac[7] = pmo * (ct2 * bemr + st * (bee - bmm));
-
+        const T loc[kNVar] = {Iee, -Iee, Iemr, Iemi, Ibee, -Ibee, Ibemr, Ibemi};
+        const T vac[kNVar] = {-(p2st * emi), p2st * emi, -(p2ct * emi), pmo * (ct2 * emr + st * (ee - mm)),
+                              -(p2st * bemi), p2st * bemi, -(p2ct * bemi), pmo * (ct2 * bemr + st * (bee - bmm))};
         T k[kNVar];
 #pragma unroll
         for (int f = 0; f < kNVar; ++f) {
-            const T xm2 = ld(yin, f, rm2 + j), xm1 = ld(yin, f, rm1 + j);
-            const T xp1 = ld(yin, f, rp1 + j), xp2 = ld(yin, f, rp2 + j);
             const T zm2 = win[f][0], zm1 = win[f][1], c0 = win[f][2], zp1 = win[f][3], zp2 = win[f][4];
             // advection (nuosc_class.cpp:182-183)
-            const T adv = fz * ((zm2 - zp2) - (T)8.0 * (zm1 - zp1)) + fx * ((xm2 - xp2) - (T)8.0 * (xm1 - xp1));
+            const T adv = fz * ((zm2 - zp2) - (T)8.0 * (zm1 - zp1)) + fx * ((xm2[f] - xp2[f]) - (T)8.0 * (xm1[f] - xp1[f]));
             // Kreiss-Oliger (nuosc_class.cpp:122-123)
             const T ko = kz * (zm2 + zp2 - (T)4 * (zm1 + zp1) + (T)6 * c0) +
-                         kx * (xm2 + xp2 - (T)4 * (xm1 + xp1) + (T)6 * c0);
+                         kx * (xm2[f] + xp2[f] - (T)4 * (xm1[f] + xp1[f]) + (T)6 * c0);
             k[f] = loc[f] + adv + ko + vac[f];
         }
 
-        T yo[kNVar];
-        if constexpr (STAGE == 0) {
+        T yo[kNVar], wn[kNVar];
 #pragma unroll
-            for (int f = 0; f < kNVar; ++f) yo[f] = k[f];
-        } else if constexpr (STAGE == 1) {
-#pragma unroll
-            for (int f = 0; f < kNVar; ++f) {
-                const T uc = win[f][2];
-                const T wn = uc + cw * k[f];
-                yo[f] = uc + cy * k[f];
-                if (act) a.w[f * fs + site * nvp + v] = wn;
-            }
-        } else if constexpr (STAGE == 2 || STAGE == 3) {
-#pragma unroll
-            for (int f = 0; f < kNVar; ++f) {
-                const T uc = ld(a.u, f, site);
-                const T wc = ld(a.w, f, site);
-                const T wn = wc + cw * k[f];
-                yo[f] = uc + cy * k[f];
-                if (act) a.w[f * fs + site * nvp + v] = wn;
-            }
-        } else {
-#pragma unroll
-            for (int f = 0; f < kNVar; ++f) {
-                const T wc = ld(a.w, f, site);
-                yo[f] = wc + cw * k[f];
+        for (int f = 0; f < kNVar; ++f) {
+            if constexpr (STAGE == 0) {
+                yo[f] = k[f];
+            } else if constexpr (STAGE == 1) {
+                wn[f] = win[f][2] + cw * k[f];
+                yo[f] = win[f][2] + cy * k[f];
+            } else if constexpr (STAGE == 2 || STAGE == 3) {
+                wn[f] = wc[f] + cw * k[f];
+                yo[f] = uc[f] + cy * k[f];
+            } else {
+                yo[f] = wc[f] + cw * k[f];
             }
         }
         if (act) {
+            const unsigned long long bw = a.w.fresh_base();
+            const unsigned long long bo = a.yout.fresh_base();
 #pragma unroll
-            for (int f = 0; f < kNVar; ++f) a.yout[f * fs + site * nvp + v] = yo[f];
+            for (int f = 0; f < kNVar; ++f) {
+                if constexpr (STAGE >= 1 && STAGE <= 3) a.w.st(bw, f, wn[f], vo, so);
+                a.yout.st(bo, f, yo[f], vo, so);
+            }
         }
         if constexpr (STAGE != 0) {
             double t[16];
             moment_terms<T>(yo, q, t);
             const double tot = wave_sum16<COMP>(t, lane);
             const int m = lane >> 2;
-            if ((lane & 3) == 0 && m < kNMom) a.mout[(site * nch + ch) * kMomStride + m] = tot;
+            if ((lane & 3) == 0 && m < kNMom)
+                a.mpart[((long long)site * nch + ch) * kMomStride + m] = tot;
         }
 #pragma unroll
         for (int f = 0; f < kNVar; ++f) {
@@ -352,7 +395,7 @@ __global__ __launch_bounds__(256) void k_stage(const StageArgs<T> a) {
 }
 
 StagePlan plan_stage(const Geometry& g, int num_cus) {
-    // enough waves to give every SIMD several (>= 16 per CU), segments long
+    // enough waves to give every SIMD several (>= 12 per CU), segments long
     // enough that the 4-row window warm-up stays a small overhead
     StagePlan p{};
     const long long rows = (long long)g.nx * g.nch;
@@ -367,18 +410,40 @@ StagePlan plan_stage(const Geometry& g, int num_cus) {
     return p;
 }
 
-template <typename T>
-hipError_t launch_stage(int stage, const Geometry& g, const Physics& ph, const StagePlan& pl,
-                        const T* yin, const T* u, T* w, T* yout, const double* min, double* mout,
-                        double cw, double cy, hipStream_t s) {
-    StageArgs<T> a;
+bool buffer_addressable(const Geometry& g, int esize) {
+    return (long long)g.fs * esize < (1ll << 32) - 1;
+}
+
+// fields per buffer descriptor: the largest of 4, 2, 1 whose group spans
+// fewer than 4 GiB
+static int fields_per_descriptor(const Geometry& g, int esize) {
+    const long long fb = g.fs * esize;
+    for (int fpd : {4, 2}) if (fb * fpd < (1ll << 32) - 1) return fpd;
+    return 1;
+}
+
+template <typename T, int FPD>
+static Fields<T, FPD> fields_of(const Geometry& g, const T* p) {
+    Fields<T, FPD> f;
+    f.base = (unsigned long long)p;
+    f.field_bytes = (unsigned)(g.fs * (long long)sizeof(T));
+    f.group_bytes = (unsigned)(g.fs * (long long)sizeof(T) * FPD);
+    f.group_stride = (unsigned long long)f.group_bytes;
+    return f;
+}
+
+template <typename T, int FPD>
+static hipError_t launch_stage_fpd(int stage, const Geometry& g, const Physics& ph, const StagePlan& pl,
+                                   const T* yin, const T* u, T* w, T* yout, const double* mtot, double* mpart,
+                                   double cw, double cy, hipStream_t s) {
+    StageArgs<T, FPD> a;
     a.g = g;
-    a.yin = yin;
-    a.u = u;
-    a.w = w;
-    a.yout = yout;
-    a.min = min;
-    a.mout = mout;
+    a.yin = fields_of<T, FPD>(g, yin);
+    a.u = fields_of<T, FPD>(g, u ? u : yin);
+    a.w = fields_of<T, FPD>(g, w ? w : yout);
+    a.yout = fields_of<T, FPD>(g, yout);
+    a.mtot = mtot;
+    a.mpart = mpart;
     a.cw = cw;
     a.cy = cy;
     a.mu2 = 2 * ph.mu;
@@ -393,13 +458,63 @@ hipError_t launch_stage(int stage, const Geometry& g, const Physics& ph, const S
     constexpr bool COMP = sizeof(T) == 4;
     dim3 grid(pl.nblocks), block(256);
     switch (stage) {
-        case 0: hipLaunchKernelGGL((k_stage<T, 0, COMP>), grid, block, 0, s, a); break;
-        case 1: hipLaunchKernelGGL((k_stage<T, 1, COMP>), grid, block, 0, s, a); break;
-        case 2: hipLaunchKernelGGL((k_stage<T, 2, COMP>), grid, block, 0, s, a); break;
-        case 3: hipLaunchKernelGGL((k_stage<T, 3, COMP>), grid, block, 0, s, a); break;
-        case 4: hipLaunchKernelGGL((k_stage<T, 4, COMP>), grid, block, 0, s, a); break;
+        case 0: hipLaunchKernelGGL((k_stage<T, FPD, 0, COMP>), grid, block, 0, s, a); break;
+        case 1: hipLaunchKernelGGL((k_stage<T, FPD, 1, COMP>), grid, block, 0, s, a); break;
+        case 2: hipLaunchKernelGGL((k_stage<T, FPD, 2, COMP>), grid, block, 0, s, a); break;
+        case 3: hipLaunchKernelGGL((k_stage<T, FPD, 3, COMP>), grid, block, 0, s, a); break;
+        case 4: hipLaunchKernelGGL((k_stage<T, FPD, 4, COMP>), grid, block, 0, s, a); break;
         default: return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_stage(int stage, const Geometry& g, const Physics& ph, const StagePlan& pl,
+                        const T* yin, const T* u, T* w, T* yout, const double* mtot, double* mpart,
+                        double cw, double cy, hipStream_t s) {
+    if (!buffer_addressable(g, sizeof(T))) return hipErrorInvalidValue;
+    switch (fields_per_descriptor(g, sizeof(T))) {
+        case 4: return launch_stage_fpd<T, 4>(stage, g, ph, pl, yin, u, w, yout, mtot, mpart, cw, cy, s);
+        case 2: return launch_stage_fpd<T, 2>(stage, g, ph, pl, yin, u, w, yout, mtot, mpart, cw, cy, s);
+        default: return launch_stage_fpd<T, 1>(stage, g, ph, pl, yin, u, w, yout, mtot, mpart, cw, cy, s);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// per-site moment totals: sum of the nch chunk records in chunk order
+// ---------------------------------------------------------------------------
+template <bool COMP>
+__global__ __launch_bounds__(256) void k_moment_totals(const double* __restrict__ part, int nch,
+                                                       long long nsites, double* __restrict__ tot) {
+    const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nsites * kMomStride) return;
+    const long long site = n / kMomStride;
+    const int s = (int)(n % kMomStride);
+    if (s >= kNMom) { tot[n] = 0.0; return; }
+    const double* rec = part + site * nch * kMomStride + s;
+    double sum = rec[0];
+    if constexpr (!COMP) {
+        for (int c = 1; c < nch; ++c) sum += rec[c * kMomStride];
+        tot[n] = sum;
+    } else {
+        double err = 0.0;
+        for (int c = 1; c < nch; ++c) {
+            double s2, e;
+            two_sum(sum, rec[c * kMomStride], s2, e);
+            sum = s2;
+            err += e;
+        }
+        tot[n] = sum + err;
+    }
+}
+
+hipError_t launch_moment_totals(const Geometry& g, bool compensated, const double* part, double* tot,
+                                hipStream_t s) {
+    const long long nsites = (long long)g.nx * g.nz;
+    const long long n = nsites * kMomStride;
+    const dim3 grid((unsigned)((n + 255) / 256)), block(256);
+    if (compensated) hipLaunchKernelGGL((k_moment_totals<true>), grid, block, 0, s, part, g.nch, nsites, tot);
+    else hipLaunchKernelGGL((k_moment_totals<false>), grid, block, 0, s, part, g.nch, nsites, tot);
     return hipGetLastError();
 }
 
