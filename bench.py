@@ -46,7 +46,29 @@ def parse():
     p.add_argument("--dz", type=float, default=0.05)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=12.0)
+    p.add_argument("--workload", default="box", choices=("box", "batched", "served"),
+                   help="box: one large run (default); batched: 64 uneven runs stepped together vs one by one; "
+                        "served: Poisson arrivals into the run server")
+    p.add_argument("--runs", type=int, default=64)
+    p.add_argument("--rate", type=float, default=2.0, help="served: arrivals per second")
+    p.add_argument("--duration", type=float, default=60.0, help="served: seconds of arrivals")
+    p.add_argument("--copy-calibration", action="store_true",
+                   help="run only a plain 1 GiB -> 1 GiB device copy (for scaling rocprofv3 byte counters)")
     return p.parse_args()
+
+
+def copy_calibration():
+    """A plain device copy of known size: reads 1 GiB and writes 1 GiB per
+    launch (torch elementwise kernel), three launches.  Its rocprofv3
+    FETCH_SIZE / WRITE_SIZE readings calibrate those counters."""
+    import torch
+    n = 1 << 28
+    a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        torch.mul(a, 1.0, out=b)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration": {"read_bytes": 4 * n, "write_bytes": 4 * n, "launches": 3}}), flush=True)
 
 
 def traffic_from_profiles(precision):
@@ -59,6 +81,120 @@ def traffic_from_profiles(precision):
         return d.get(f"fp{precision}", {}).get("stage2_bytes")
     except Exception:
         return None
+
+
+def scan_run(rng, k, grids=(64, 128, 192, 256), dz=0.05):
+    """One parameter-scan run: grid and direction count drawn from the scan
+    sets, mu ~ U[0.5,2], alpha ~ U[0.7,1], sigma_nu ~ U[0.4,0.8],
+    sigma_nubar = sigma_nu - 0.1, eps = 1e-6, own seed."""
+    n = int(rng.choice(grids))
+    nn = int(rng.choice((4, 8)))
+    half = n * dz / 2
+    sig = float(rng.uniform(0.4, 0.8))
+    geo = dict(nv=nn, x0=-half, x1=half, z0=-half, z1=half, dx=dz, dz=dz, CFL=0.4, ko=0.1,
+               mu=float(rng.uniform(0.5, 2.0)), pmo=0.0)
+    init = dict(ipt=5, alpha=float(rng.uniform(0.7, 1.0)), eps0=1e-6, sigma=1.0, lnue=sig, lnueb=sig - 0.1,
+                seed=1000 + k)
+    return geo, init
+
+
+def bench_batched(a):
+    """The same runs stepped one after another (each replaying its own graph)
+    and all together in one batch (one launch per stage for all runs)."""
+    import numpy as np
+
+    import nuosc_amd as N
+    prec = N.PREC_FP64 if a.precision == 64 else N.PREC_FP32
+    rng = np.random.default_rng(7)
+    specs = [scan_run(rng, k) for k in range(a.runs)]
+    sims = []
+    for geo, init in specs:
+        g = N.NuOsc(device=0, precision=prec, use_graph=True, **geo)
+        g.fillInitValue(**init)
+        sims.append(g)
+    upd = sum(g.nx * g.nz * g.nv for g in sims)
+    # one by one
+    for g in sims:
+        g.step_rk4(1)
+        g.synchronize()
+    t0 = time.perf_counter()
+    for g in sims:
+        g.step_rk4(a.steps)
+        g.synchronize()
+    t_seq = time.perf_counter() - t0
+    # together
+    b = N.Batch(device=0, precision=prec)
+    for g in sims:
+        b.add(g)
+    b.step(1)
+    t0 = time.perf_counter()
+    b.step(a.steps)
+    t_bat = time.perf_counter() - t0
+    ok = all(g.analysis().status == 0 for g in sims)
+    out = {
+        "metric": "site*dir RK4 updates/s (batched scan)",
+        "value": upd * a.steps / t_bat,
+        "unit": "updates/s",
+        "sequential_value": upd * a.steps / t_seq,
+        "speedup_vs_sequential": t_seq / t_bat,
+        "n_gpus": 1, "steps": a.steps, "runs": a.runs,
+        "dtype": "f64" if a.precision == 64 else "f32",
+        "config": {"workload": f"{a.runs} runs, grids in 64^2..256^2, 162/642 directions, uneven physics",
+                   "updates_per_step": upd},
+        "state_ok": ok,
+    }
+    b.close()
+    for g in sims:
+        g.close()
+    print(json.dumps(out), flush=True)
+
+
+def bench_served(a):
+    """Poisson arrivals into the run server; latency percentiles."""
+    import numpy as np
+
+    import nuosc_amd as N
+    prec = N.PREC_FP64 if a.precision == 64 else N.PREC_FP32
+    rng = np.random.default_rng(11)
+    srv = N.Server(device=0, precision=prec, hbm_budget_bytes=200 << 30, max_tick_steps=10)
+    t_start = time.perf_counter()
+    n, k = 0, 0
+    next_t = rng.exponential(1.0 / a.rate)
+    submitted = {}
+    while next_t < a.duration:
+        now = time.perf_counter() - t_start
+        if now < next_t:
+            time.sleep(min(0.05, next_t - now))
+            continue
+        geo, init = scan_run(rng, k, grids=(64, 128, 192, 256, 320, 384))
+        nsteps = int(rng.integers(a.steps, 10 * a.steps + 1))
+        rid = srv.submit(nsteps=nsteps, ana_every=max(1, nsteps // 20), **geo, **init)
+        submitted[rid] = nsteps
+        k += 1
+        next_t += rng.exponential(1.0 / a.rate)
+    lat, queue_wait, bad = [], [], 0
+    for _ in submitted:
+        r = srv.take(timeout=3600)
+        info, series = r
+        bad += info.status != 0
+        lat.append(info.t_done - info.t_submit)
+        queue_wait.append(info.t_admit - info.t_submit)
+    st = srv.stats()
+    srv.close()
+    wall = time.perf_counter() - t_start
+    out = {
+        "metric": "served scan: request latency and aggregate updates/s",
+        "value": st.updates / wall,
+        "unit": "updates/s",
+        "requests": len(submitted), "failed": bad, "rate_per_s": a.rate, "arrival_window_s": a.duration,
+        "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99)),
+        "queue_p50_s": float(np.percentile(queue_wait, 50)), "queue_p99_s": float(np.percentile(queue_wait, 99)),
+        "gpu_busy_frac": st.busy_s / wall,
+        "dtype": "f64" if a.precision == 64 else "f32",
+        "config": {"workload": f"Poisson {a.rate}/s for {a.duration}s, grids 64^2..384^2, 162/642 directions, "
+                               f"steps U[{a.steps},{10 * a.steps}], 200 GB budget"},
+    }
+    print(json.dumps(out), flush=True)
 
 
 def cpu_baseline(nn, dz, seconds):
@@ -92,6 +228,15 @@ def cpu_baseline(nn, dz, seconds):
 
 def main():
     a = parse()
+    if a.copy_calibration:
+        copy_calibration()
+        return
+    if a.workload == "batched":
+        bench_batched(a)
+        return
+    if a.workload == "served":
+        bench_served(a)
+        return
     if a.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {a.gpus}); "
               "the periodic box would shard as an (x,z) domain decomposition with radius-2 halos.",

@@ -40,6 +40,7 @@ extern "C" {
 #define NUOSC_ENOMEM 3      /* device or host allocation failed         */
 #define NUOSC_EUNSTABLE 4   /* analysis: maxdP >= 10 or non-finite state */
 #define NUOSC_ENODEV 5      /* no usable GPU                            */
+#define NUOSC_EAGAIN 6      /* server: no result ready within the wait  */
 
 #define NUOSC_PREC_FP64 64
 #define NUOSC_PREC_FP32 32
@@ -177,6 +178,78 @@ int nuosc_last_step_ms(nuosc_sim* sim, double* ms);
  * duration of each stage kernel in stage_ms[4] (ms).  Advances the state and
  * phy_time like nuosc_step_rk4. */
 int nuosc_profile_stages(nuosc_sim* sim, int nsteps, double* stage_ms);
+
+/* ------------------------------------------------------------------------
+ * Batched runs (no reference counterpart: the reference runs one NuOsc per
+ * process).  A batch advances many independent handles -- different grids,
+ * direction counts and physics -- with one launch per RK4 stage for all of
+ * them.  Members keep their own state and remain usable through the
+ * single-run functions between batch steps; a member's results are bitwise
+ * the same stepped alone or in a batch.  All members share the batch's
+ * device and precision; renormalising members are not supported.
+ * ------------------------------------------------------------------------ */
+typedef struct nuosc_batch nuosc_batch;
+int nuosc_batch_create(int device, int precision, nuosc_batch** out);
+int nuosc_batch_destroy(nuosc_batch* batch);
+int nuosc_batch_add(nuosc_batch* batch, nuosc_sim* sim);
+int nuosc_batch_remove(nuosc_batch* batch, nuosc_sim* sim);
+int nuosc_batch_size(const nuosc_batch* batch);
+/* nsteps RK4 steps of every member (phy_time += dt per step each);
+ * returns after the device work completed. */
+int nuosc_batch_step(nuosc_batch* batch, int nsteps);
+const char* nuosc_batch_last_error(const nuosc_batch* batch);
+
+/* Host-only: device bytes a handle created from cfg would hold (admission
+ * control for the serving loop). */
+int nuosc_estimate_device_bytes(const nuosc_config* cfg, long long* bytes);
+
+/* ------------------------------------------------------------------------
+ * Run server (no reference counterpart).  Parameter-scan runs are submitted
+ * by any host thread; one dispatcher thread admits them in arrival order
+ * while their summed device footprint fits the HBM budget, advances all
+ * resident runs together through a batch (continuous batching: runs join and
+ * leave between ticks of at most max_tick_steps steps) and records each run's
+ * (t, Pee, Peebar, maxdP, |M0|) series every ana_every steps.
+ * ------------------------------------------------------------------------ */
+typedef struct nuosc_server nuosc_server;
+
+typedef struct nuosc_request {
+    nuosc_config cfg;   /* geometry and physics (device/precision from the server) */
+    int ipt;
+    double alpha, eps0, sigma, lnue, lnueb, lnuex, lnuebx;
+    uint64_t seed;
+    int nsteps;
+    int ana_every;
+} nuosc_request;
+
+typedef struct nuosc_result {
+    long long id;
+    int status;          /* 0 or the NUOSC_E* code that stopped the run */
+    int nsteps;          /* steps done */
+    double t_submit, t_admit, t_done;   /* seconds on the server's clock */
+    long long device_bytes;
+    int n_samples;       /* series rows of 5 doubles: t, Pee, Peebar, maxdP, |M0| */
+} nuosc_result;
+
+typedef struct nuosc_server_stats_t {
+    int queued, active;
+    long long finished;
+    long long resident_bytes;
+    double updates;      /* site*dir RK4 updates done so far */
+    double busy_s;       /* dispatcher time inside batch steps */
+    double now_s;
+} nuosc_server_stats_t;
+
+int nuosc_server_create(int device, int precision, long long hbm_budget_bytes, int max_tick_steps,
+                        nuosc_server** out);
+int nuosc_server_submit(nuosc_server* srv, const nuosc_request* req, long long* id);
+/* id < 0: any finished run.  Waits up to timeout_s; NUOSC_EAGAIN if none.
+ * series (optional) receives min(n_samples, max_samples) rows. */
+int nuosc_server_take_result(nuosc_server* srv, long long id, double timeout_s, nuosc_result* info,
+                             double* series, int max_samples);
+int nuosc_server_stats(nuosc_server* srv, nuosc_server_stats_t* stats);
+/* Stops the dispatcher: queued and resident runs are cancelled. */
+int nuosc_server_destroy(nuosc_server* srv);
 
 #ifdef __cplusplus
 }

@@ -24,7 +24,7 @@ HEADER = os.path.join(os.path.dirname(HERE), "include", "nuosc_hip.h")
 FIELDS = ("ee", "mm", "emr", "emi", "bee", "bmm", "bemr", "bemi")
 DBL_MAX = np.finfo(np.float64).max
 
-NUOSC_OK, NUOSC_EINVAL, NUOSC_EHIP, NUOSC_ENOMEM, NUOSC_EUNSTABLE, NUOSC_ENODEV = range(6)
+NUOSC_OK, NUOSC_EINVAL, NUOSC_EHIP, NUOSC_ENOMEM, NUOSC_EUNSTABLE, NUOSC_ENODEV, NUOSC_EAGAIN = range(7)
 PREC_FP64, PREC_FP32 = 64, 32
 VGRID_VORONOI, VGRID_EQUAL = 0, 1
 
@@ -43,6 +43,30 @@ class Config(C.Structure):
         ("cfl", C.c_double), ("ko", C.c_double), ("mu", C.c_double), ("pmo", C.c_double),
         ("renorm", C.c_int), ("vgrid", C.c_int), ("precision", C.c_int), ("bc", C.c_int),
         ("device", C.c_int), ("use_graph", C.c_int),
+    ]
+
+
+class Request(C.Structure):
+    _fields_ = [
+        ("cfg", Config), ("ipt", C.c_int),
+        ("alpha", C.c_double), ("eps0", C.c_double), ("sigma", C.c_double), ("lnue", C.c_double),
+        ("lnueb", C.c_double), ("lnuex", C.c_double), ("lnuebx", C.c_double),
+        ("seed", C.c_uint64), ("nsteps", C.c_int), ("ana_every", C.c_int),
+    ]
+
+
+class Result(C.Structure):
+    _fields_ = [
+        ("id", C.c_longlong), ("status", C.c_int), ("nsteps", C.c_int),
+        ("t_submit", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double),
+        ("device_bytes", C.c_longlong), ("n_samples", C.c_int),
+    ]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [
+        ("queued", C.c_int), ("active", C.c_int), ("finished", C.c_longlong), ("resident_bytes", C.c_longlong),
+        ("updates", C.c_double), ("busy_s", C.c_double), ("now_s", C.c_double),
     ]
 
 
@@ -96,6 +120,19 @@ SIGNATURES = {
     "nuosc_p3_slice": ([_vp, _ip, C.c_int, _ip, C.c_int, _dp], C.c_int),
     "nuosc_last_step_ms": ([_vp, _dp], C.c_int),
     "nuosc_profile_stages": ([_vp, C.c_int, _dp], C.c_int),
+    "nuosc_batch_create": ([C.c_int, C.c_int, C.POINTER(_vp)], C.c_int),
+    "nuosc_batch_destroy": ([_vp], C.c_int),
+    "nuosc_batch_add": ([_vp, _vp], C.c_int),
+    "nuosc_batch_remove": ([_vp, _vp], C.c_int),
+    "nuosc_batch_size": ([_vp], C.c_int),
+    "nuosc_batch_step": ([_vp, C.c_int], C.c_int),
+    "nuosc_batch_last_error": ([_vp], C.c_char_p),
+    "nuosc_estimate_device_bytes": ([C.POINTER(Config), C.POINTER(C.c_longlong)], C.c_int),
+    "nuosc_server_create": ([C.c_int, C.c_int, C.c_longlong, C.c_int, C.POINTER(_vp)], C.c_int),
+    "nuosc_server_submit": ([_vp, C.POINTER(Request), C.POINTER(C.c_longlong)], C.c_int),
+    "nuosc_server_take_result": ([_vp, C.c_longlong, C.c_double, C.POINTER(Result), _dp, C.c_int], C.c_int),
+    "nuosc_server_stats": ([_vp, C.POINTER(ServerStats)], C.c_int),
+    "nuosc_server_destroy": ([_vp], C.c_int),
 }
 
 
@@ -300,3 +337,108 @@ class NuOsc:
         self._check(lib().nuosc_p3_slice(self._h, xi.ctypes.data_as(_ip), len(xi),
                                          vi.ctypes.data_as(_ip), len(vi), _ptr(out)))
         return out.reshape(len(xi), self.nz, len(vi))
+
+
+class Batch:
+    """Many independent NuOsc runs advanced together, one launch per RK4
+    stage for all of them (nuosc_batch_* in include/nuosc_hip.h)."""
+
+    def __init__(self, device=0, precision=PREC_FP64):
+        h = _vp()
+        rc = lib().nuosc_batch_create(device, precision, C.byref(h))
+        if rc != 0:
+            raise NuOscError(rc, (lib().nuosc_last_error(None) or b"").decode())
+        self._h = h
+        self.members = []
+
+    def _check(self, rc):
+        if rc != 0:
+            raise NuOscError(rc, (lib().nuosc_batch_last_error(self._h) or b"").decode())
+
+    def add(self, sim):
+        self._check(lib().nuosc_batch_add(self._h, sim._h))
+        self.members.append(sim)
+
+    def remove(self, sim):
+        self._check(lib().nuosc_batch_remove(self._h, sim._h))
+        self.members.remove(sim)
+
+    def __len__(self):
+        return lib().nuosc_batch_size(self._h)
+
+    def step(self, nsteps=1):
+        self._check(lib().nuosc_batch_step(self._h, int(nsteps)))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().nuosc_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def estimate_device_bytes(nv=4, x0=-1.0, x1=1.0, z0=-10.0, z1=10.0, dx=0.1, dz=0.1, precision=PREC_FP64, **_):
+    cfg = Config(nn=nv, gx=2, gz=2, x0=x0, x1=x1, z0=z0, z1=z1, dx=dx, dz=dz, precision=precision)
+    b = C.c_longlong()
+    rc = lib().nuosc_estimate_device_bytes(C.byref(cfg), C.byref(b))
+    if rc != 0:
+        raise NuOscError(rc, "bad configuration")
+    return b.value
+
+
+class Server:
+    """Continuous-batching run server (nuosc_server_* in include/nuosc_hip.h).
+
+    submit() queues a run and returns its id; results() yields finished runs
+    as (Result, series) with series rows (t, Pee, Peebar, maxdP, |M0|)."""
+
+    def __init__(self, device=0, precision=PREC_FP64, hbm_budget_bytes=200 << 30, max_tick_steps=10):
+        h = _vp()
+        rc = lib().nuosc_server_create(device, precision, int(hbm_budget_bytes), int(max_tick_steps), C.byref(h))
+        if rc != 0:
+            raise NuOscError(rc, "server creation failed")
+        self._h = h
+
+    def submit(self, nsteps, ana_every=10, ipt=5, alpha=0.9, eps0=1e-6, sigma=1.0, lnue=0.6, lnueb=0.5,
+               lnuex=DBL_MAX, lnuebx=DBL_MAX, seed=0, nv=4, x0=-1.0, x1=1.0, z0=-1.0, z1=1.0, dx=0.1, dz=0.1,
+               CFL=0.4, ko=0.1, mu=1.0, pmo=0.0):
+        cfg = Config(nn=nv, gx=2, gz=2, x0=x0, x1=x1, z0=z0, z1=z1, dx=dx, dz=dz, cfl=CFL, ko=ko, mu=mu, pmo=pmo,
+                     renorm=0, vgrid=0, precision=0, bc=0, device=0, use_graph=0)
+        req = Request(cfg=cfg, ipt=ipt, alpha=alpha, eps0=eps0, sigma=sigma, lnue=lnue, lnueb=lnueb, lnuex=lnuex,
+                      lnuebx=lnuebx, seed=seed, nsteps=nsteps, ana_every=ana_every)
+        rid = C.c_longlong()
+        rc = lib().nuosc_server_submit(self._h, C.byref(req), C.byref(rid))
+        if rc != 0:
+            raise NuOscError(rc, "submit rejected")
+        return rid.value
+
+    def take(self, rid=-1, timeout=60.0, max_samples=100000):
+        info = Result()
+        buf = np.zeros(5 * max_samples)
+        rc = lib().nuosc_server_take_result(self._h, int(rid), float(timeout), C.byref(info), _ptr(buf), max_samples)
+        if rc == NUOSC_EAGAIN:
+            return None
+        if rc != 0:
+            raise NuOscError(rc, "take_result failed")
+        n = min(info.n_samples, max_samples)
+        return info, buf[:5 * n].reshape(n, 5).copy()
+
+    def stats(self):
+        st = ServerStats()
+        lib().nuosc_server_stats(self._h, C.byref(st))
+        return st
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().nuosc_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

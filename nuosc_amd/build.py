@@ -37,7 +37,7 @@ HOST_FLAGS = [
 ]
 
 KERNEL_SRCS = ["nuosc_kernels.hip"]
-HOST_SRCS = ["nuosc_capi.cpp", "vgrid.cpp"]
+HOST_SRCS = ["nuosc_capi.cpp", "vgrid.cpp", "nuosc_serve.cpp"]
 CLI_SRCS = ["NuOsc.cpp", "nuosc_main.cpp"]
 
 
@@ -61,7 +61,7 @@ def build(verbose: bool = True) -> str:
         objs.append(obj)
     tmp = LIB + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp,
-          f"-Wl,-rpath,{ROCM}/lib"])
+          f"-Wl,-rpath,{ROCM}/lib", "-lpthread"])
     os.replace(tmp, LIB)
     # command-line driver linked against the library
     if all(os.path.exists(os.path.join(CSRC, s)) for s in CLI_SRCS):

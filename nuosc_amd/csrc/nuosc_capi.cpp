@@ -226,7 +226,8 @@ int nuosc_create(const nuosc_config* cfg, nuosc_sim** out) {
     s->esize = is64(s) ? 8 : 4;
     const int align = (int)(128 / s->esize);
     s->nvp = (s->nv + align - 1) / align * align;
-    s->nch = (s->nv + kWave - 1) / kWave;
+    const int lpr = kWave / rows_per_wave((int)s->esize);
+    s->nch = (s->nv + lpr - 1) / lpr;
     s->fs = (long long)s->nx * s->nz * s->nvp;
     if (s->fs * (long long)s->esize >= (1ll << 32) - 1) {
         delete s;
@@ -252,7 +253,7 @@ int nuosc_create(const nuosc_config* cfg, nuosc_sim** out) {
         return bail(NUOSC_EHIP, "event creation failed");
 
     const size_t state_bytes = (size_t)kNVar * s->fs * s->esize;
-    const size_t part_bytes = (size_t)s->nx * s->nz * s->nch * kMomStride * sizeof(double);
+    const size_t part_bytes = (size_t)s->nx * s->nz * s->nch * kMomRec * sizeof(double);
     const size_t mom_bytes = (size_t)s->nx * s->nz * kMomStride * sizeof(double);
     void** states[4] = {&s->st_u, &s->st_w, &s->st_ya, &s->st_yb};
     for (void** p : states) {
@@ -276,6 +277,7 @@ int nuosc_create(const nuosc_config* cfg, nuosc_sim** out) {
     s->geo.nv = s->nv;
     s->geo.nvp = s->nvp;
     s->geo.nch = s->nch;
+    s->geo.lpr = lpr;
     s->geo.fs = s->fs;
     s->geo.twelve_dx = 12 * c.dx;
     s->geo.twelve_dz = 12 * c.dz;
@@ -671,6 +673,164 @@ int nuosc_p3_slice(nuosc_sim* s, const int* xidx, int sx, const int* vidx, int s
     else HIPCK(s, launch_p3_slice<float>(s->geo, (float*)s->st_u, s->d_g0, di, sx, di + sx, sv, dd, s->stream));
     HIPCK(s, hipMemcpyAsync(dst, dd, sizeof(double) * n, hipMemcpyDeviceToHost, s->stream));
     HIPCK(s, hipStreamSynchronize(s->stream));
+    return NUOSC_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// batched runs
+// ---------------------------------------------------------------------------
+struct nuosc_batch {
+    int device = 0;
+    int precision = NUOSC_PREC_FP64;
+    std::vector<nuosc_sim*> members;
+    hipStream_t stream = nullptr;
+    BatchTables tables;
+    bool dirty = true;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    std::string err;
+};
+
+namespace {
+int bfail(nuosc_batch* b, int code, const std::string& m) {
+    b->err = m;
+    return code;
+}
+void batch_drop_graph(nuosc_batch* b) {
+    if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->graph_exec = nullptr;
+    b->graph = nullptr;
+}
+}  // namespace
+
+extern "C" {
+
+int nuosc_batch_create(int device, int precision, nuosc_batch** out) {
+    if (!out) return NUOSC_EINVAL;
+    *out = nullptr;
+    if (precision != NUOSC_PREC_FP64 && precision != NUOSC_PREC_FP32)
+        return fail(nullptr, NUOSC_EINVAL, "precision must be 64 or 32");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(nullptr, NUOSC_ENODEV, "no HIP device");
+    if (device < 0 || device >= ndev) return fail(nullptr, NUOSC_EINVAL, "device ordinal out of range");
+    if (hipSetDevice(device) != hipSuccess) return fail(nullptr, NUOSC_EHIP, "hipSetDevice failed");
+    nuosc_batch* b = new nuosc_batch();
+    b->device = device;
+    b->precision = precision;
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete b;
+        return fail(nullptr, NUOSC_EHIP, "stream creation failed");
+    }
+    *out = b;
+    return NUOSC_OK;
+}
+
+int nuosc_batch_destroy(nuosc_batch* b) {
+    if (!b) return NUOSC_OK;
+    (void)hipStreamSynchronize(b->stream);
+    batch_drop_graph(b);
+    batch_free(b->tables);
+    (void)hipStreamDestroy(b->stream);
+    delete b;
+    return NUOSC_OK;
+}
+
+const char* nuosc_batch_last_error(const nuosc_batch* b) { return b ? b->err.c_str() : g_create_error.c_str(); }
+
+int nuosc_batch_add(nuosc_batch* b, nuosc_sim* s) {
+    if (!b || !s) return NUOSC_EINVAL;
+    if (s->cfg.device != b->device || s->cfg.precision != b->precision)
+        return bfail(b, NUOSC_EINVAL, "member must use the batch's device and precision");
+    if (s->cfg.renorm) return bfail(b, NUOSC_EINVAL, "renormalising runs cannot be batched");
+    for (nuosc_sim* m : b->members)
+        if (m == s) return bfail(b, NUOSC_EINVAL, "already a member");
+    b->members.push_back(s);
+    b->dirty = true;
+    return NUOSC_OK;
+}
+
+int nuosc_batch_remove(nuosc_batch* b, nuosc_sim* s) {
+    if (!b || !s) return NUOSC_EINVAL;
+    for (size_t k = 0; k < b->members.size(); ++k)
+        if (b->members[k] == s) {
+            (void)hipStreamSynchronize(b->stream);
+            b->members.erase(b->members.begin() + k);
+            b->dirty = true;
+            return NUOSC_OK;
+        }
+    return bfail(b, NUOSC_EINVAL, "not a member");
+}
+
+int nuosc_batch_size(const nuosc_batch* b) { return b ? (int)b->members.size() : 0; }
+
+int nuosc_batch_step(nuosc_batch* b, int nsteps) {
+    if (!b || nsteps < 0) return NUOSC_EINVAL;
+    if (b->members.empty() || nsteps == 0) return NUOSC_OK;
+    // members' own queued work (init, uploads, moments) completes first
+    for (nuosc_sim* m : b->members) {
+        if (m->cfg.renorm) return bfail(b, NUOSC_EINVAL, "renormalising runs cannot be batched");
+        if (!m->have_moments && refresh_moments(m) != NUOSC_OK) return bfail(b, NUOSC_EHIP, m->err);
+        if (hipStreamSynchronize(m->stream) != hipSuccess) return bfail(b, NUOSC_EHIP, "member stream sync failed");
+    }
+    if (b->dirty) {
+        batch_drop_graph(b);
+        std::vector<BatchMember> ms;
+        for (nuosc_sim* m : b->members) {
+            BatchMember bm;
+            bm.g = m->geo;
+            bm.ph = m->ph;
+            bm.pl = m->plan;
+            bm.u = m->st_u;
+            bm.w = m->st_w;
+            bm.ya = m->st_ya;
+            bm.yb = m->st_yb;
+            bm.mpart = m->mom_part;
+            bm.mu = m->mom_u;
+            bm.ma = m->mom_a;
+            bm.mb = m->mom_b;
+            bm.dt = m->dt;
+            ms.push_back(bm);
+        }
+        hipError_t e = batch_build(b->tables, ms, is64(b->members[0]) ? 8 : 4, b->stream);
+        if (e != hipSuccess) return bfail(b, NUOSC_EHIP, std::string("batch tables: ") + hipGetErrorString(e));
+        if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+            return bfail(b, NUOSC_EHIP, "capture failed");
+        hipError_t le = batch_step(b->tables, b->stream);
+        hipGraph_t gr = nullptr;
+        hipError_t ce = hipStreamEndCapture(b->stream, &gr);
+        if (le != hipSuccess || ce != hipSuccess) {
+            if (gr) (void)hipGraphDestroy(gr);
+            return bfail(b, NUOSC_EHIP, "batched step capture failed");
+        }
+        b->graph = gr;
+        if (hipGraphInstantiate(&b->graph_exec, b->graph, nullptr, nullptr, 0) != hipSuccess)
+            return bfail(b, NUOSC_EHIP, "graph instantiation failed");
+        b->dirty = false;
+    }
+    for (int n = 0; n < nsteps; ++n) {
+        if (hipGraphLaunch(b->graph_exec, b->stream) != hipSuccess) return bfail(b, NUOSC_EHIP, "graph launch failed");
+        for (nuosc_sim* m : b->members) m->phy_time += m->dt;
+    }
+    if (hipStreamSynchronize(b->stream) != hipSuccess) return bfail(b, NUOSC_EHIP, "batch stream sync failed");
+    return NUOSC_OK;
+}
+
+int nuosc_estimate_device_bytes(const nuosc_config* c, long long* bytes) {
+    if (!c || !bytes || c->nn < 1 || c->nn > 64 || !(c->dx > 0) || !(c->dz > 0)) return NUOSC_EINVAL;
+    const long long nx = (long long)((c->x1 - c->x0) / c->dx), nz = (long long)((c->z1 - c->z0) / c->dz);
+    if (nx < 1 || nz < 1) return NUOSC_EINVAL;
+    const long long nv = 10ll * c->nn * c->nn + 2;
+    const int esize = c->precision == NUOSC_PREC_FP32 ? 4 : 8;
+    const long long align = 128 / esize, nvp = (nv + align - 1) / align * align;
+    const long long lpr = kWave / rows_per_wave(esize), nch = (nv + lpr - 1) / lpr;
+    long long b = 4ll * kNVar * nx * nz * nvp * esize;          // u, w, yA, yB
+    b += nx * nz * nch * kMomRec * 8 + 3ll * nx * nz * kMomStride * 8;  // moments
+    b += 4 * nv * 8 + 2 * nv * 8 + 2048ll * kNAna * 16 + kNAna * 8;    // v-grid, G0, analysis
+    b += 2 * nz * 8;                                                   // init tables
+    *bytes = b;
     return NUOSC_OK;
 }
 

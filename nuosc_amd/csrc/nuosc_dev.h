@@ -9,18 +9,20 @@
 //   Velocity moments: 12 doubles {A0..A3, B0..B3, C0..C3} per site with
 //   A = sum w u (bemr-emr), B = sum w u (bemi+emi), C = sum w u
 //   (bee-bmm-ee+mm), u = (1, vx, vy, vz).  The stage kernels write partial
-//   records [nx*nz sites][nch chunks][16] (each summed over the 64 directions
+//   records [nx*nz sites][nch chunks][12] (each summed over the lpr directions
 //   of one chunk); k_moment_totals adds them in chunk order into totals
 //   [nx*nz sites][16], which the next stage reads.
 #pragma once
 #include <hip/hip_runtime_api.h>
 #include <cstdint>
+#include <vector>
 
 namespace nuosc {
 
 constexpr int kNVar = 8;
 constexpr int kWave = 64;
-constexpr int kMomStride = 16;   // doubles per (site, chunk) moment record
+constexpr int kMomStride = 16;   // doubles per per-site moment total record (12 used)
+constexpr int kMomRec = 12;      // doubles per (site, chunk) partial moment record
 constexpr int kNMom = 12;
 constexpr int kNAna = 12;        // analysis accumulators (see AnaIdx)
 
@@ -30,7 +32,9 @@ enum AnaIdx {
 };
 
 struct Geometry {
-    int nx, nz, nv, nvp, nch;
+    int nx, nz, nv, nvp;
+    int lpr;               // directions per row segment of a wavefront (64 / rows per wave)
+    int nch;               // direction chunks per site = ceil(nv / lpr)
     long long fs;          // elements per field
     double twelve_dx, twelve_dz;   // 12*dx, 12*dz (advection denominators)
     double ko_x, ko_z;     // -ko/dx/16, -ko/dz/16
@@ -50,8 +54,12 @@ struct Physics {
 // mtot holds the moment totals of yin; mpart (stages 1-4) receives the partial
 // moments of yout.
 struct StagePlan {
-    int zlen, nzs, ntask, nblocks;
+    int zlen, nzs, nquads, ntask, nblocks;
 };
+
+// x-rows per wavefront of the stage kernel (FP64: 4 rows x 16 directions,
+// FP32: 2 rows x 32 directions)
+int rows_per_wave(int esize);
 
 StagePlan plan_stage(const Geometry& g, int num_cus);
 // The stage kernel addresses each field through a buffer descriptor: one
@@ -101,6 +109,29 @@ hipError_t launch_unpack(const Geometry& g, const double* src, T* field, hipStre
 template <typename T>
 hipError_t launch_p3_slice(const Geometry& g, const T* y, const double* G0, const int* xi, int sx,
                            const int* vi, int sv, double* dst, hipStream_t s);
+
+// Batched runs: every member run advances one RK4 step per batch_step call
+// with one launch per stage for all members (plus one moment-total launch per
+// stage).  Members keep their own buffers, plans and physics; all share the
+// precision (esize) of the tables.
+struct BatchMember {
+    Geometry g;
+    Physics ph;
+    StagePlan pl;
+    void *u, *w, *ya, *yb;          // state copies
+    double *mpart, *mu, *ma, *mb;   // partial records; totals of u, yA, yB
+    double dt;
+};
+struct BatchTables {
+    void* args[4] = {nullptr, nullptr, nullptr, nullptr};  // per-stage StageArgs arrays
+    void* tot[4] = {nullptr, nullptr, nullptr, nullptr};   // per-stage moment-total arrays
+    int* bmap = nullptr;   // logical block -> (member, block of member)
+    int* tmap = nullptr;
+    int nblocks = 0, tblocks = 0, fpd = 1, esize = 8;
+};
+hipError_t batch_build(BatchTables& t, const std::vector<BatchMember>& members, int esize, hipStream_t s);
+void batch_free(BatchTables& t);
+hipError_t batch_step(const BatchTables& t, hipStream_t s);
 
 // Renormalisation of the state to |P| = 1 (the reference's formula).
 template <typename T>

@@ -27,7 +27,9 @@
 
 #include "nuosc_dev.h"
 
+#include <algorithm>
 #include <cstdint>
+#include <vector>
 
 namespace nuosc {
 
@@ -46,6 +48,15 @@ __device__ __forceinline__ int xcd_remap(int b, int nb) {
     const int per = nb / 8, rem = nb % 8;
     const int x = b % 8, loc = b / 8;
     return (x < rem) ? x * (per + 1) + loc : rem * (per + 1) + (x - rem) * per + loc;
+}
+
+// Orders the LDS accesses of a wavefront: everything before is visible to
+// every lane of the wave after (the wave executes in lockstep; this keeps the
+// compiler from moving LDS accesses across it).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // Knuth/Dekker two-sum: s + e == a + b exactly.
@@ -152,6 +163,7 @@ __device__ __forceinline__ void moment_terms(const T (&y)[kNVar], const double (
 // per-load address arithmetic lives in VGPRs
 // ---------------------------------------------------------------------------
 typedef __attribute__((ext_vector_type(2))) unsigned int u32x2;
+typedef __attribute__((ext_vector_type(4))) unsigned int u32x4;
 constexpr int kRsrcWord3 = 0x00020000;  // gfx9 raw-buffer descriptor word 3
 
 template <typename T>
@@ -213,35 +225,144 @@ typedef const __attribute__((address_space(4))) double cdouble;
 
 // ---------------------------------------------------------------------------
 // RK4 stage kernel
+//
+// A wavefront covers R adjacent x-rows x L = 64/R directions (FP64: 4 x 16,
+// FP32: 2 x 32, so every row segment is one 128-byte line) and marches along
+// z over one segment.  Per lane it keeps a 5-deep z-window of its own row and
+// a 3-deep FIFO of the halo rows (i0-2, i0-1, i0+R, i0+R+1), both for all 8
+// fields.  Halo rows are loaded at z+2, in the same iteration in which the
+// neighbouring waves load those rows as their own window rows, so the second
+// read hits in L2 instead of HBM.  The x-neighbours are exchanged through a
+// wave-private LDS slab (no barrier: one wave's LDS operations complete in
+// order).
 // ---------------------------------------------------------------------------
 template <typename T, int FPD>
 struct StageArgs {
     Geometry g;
     Fields<T, FPD> yin, u, w, yout;
-    const double* mtot;     // moment totals of yin: [site][kMomStride]
-    double* mpart;          // partial moments of yout: [site][nch][kMomStride]
+    unsigned long long mtot;  // moment totals of yin: [site][kMomStride] doubles
+    unsigned mtot_bytes;
+    double* mpart;            // partial moments of yout: [site][nch][kMomRec]
     double cw, cy;
     double mu2, mu, p2st, p2ct, pmo, ct2, st;
-    int zlen, ntask;
+    int zlen, ntask, nquads;
 };
 
-template <typename T, int FPD, int STAGE, bool COMP>
-__global__ __launch_bounds__(256) void k_stage(const StageArgs<T, FPD> a) {
+// Sums of 16 values over each group of L consecutive lanes (L = 16, 32, 64):
+// four halving exchanges (each lane keeps the half of the values selected by
+// one lane-index bit) and plain butterflies for the remaining bits.  Lane d of
+// a group ends with the group total of value d / (L/16).  Each level has its
+// own arrays so that every select has constant indices.
+template <int M>
+__device__ __forceinline__ void halve(const double* in, double* out, int n2, bool hb) {
+#pragma unroll
+    for (int k = 0; k < n2; ++k) {
+        const double lo = in[k], hi = in[k + n2];
+        out[k] = (hb ? hi : lo) + __shfl_xor(hb ? lo : hi, M);
+    }
+}
+template <int M>
+__device__ __forceinline__ void halve_c(const double* in, const double* ine, double* out, double* oute, int n2,
+                                        bool hb) {
+#pragma unroll
+    for (int k = 0; k < n2; ++k) {
+        const double lo = in[k], hi = in[k + n2], loe = ine[k], hie = ine[k + n2];
+        double s2, err;
+        two_sum(hb ? hi : lo, __shfl_xor(hb ? lo : hi, M), s2, err);
+        out[k] = s2;
+        oute[k] = (hb ? hie : loe) + __shfl_xor(hb ? loe : hie, M) + err;
+    }
+}
+
+template <int L, bool COMP>
+__device__ __forceinline__ double group_sum16(const double (&a)[16], int lane) {
+    constexpr int M1 = L / 2, M2 = L / 4, M3 = L / 8, M4 = L / 16;
+    if constexpr (!COMP) {
+        double b[8], c[4], d[2], e[1];
+        halve<M1>(a, b, 8, lane & M1);
+        halve<M2>(b, c, 4, lane & M2);
+        halve<M3>(c, d, 2, lane & M3);
+        halve<M4>(d, e, 1, lane & M4);
+        double x = e[0];
+#pragma unroll
+        for (int m = M4 / 2; m >= 1; m >>= 1) x += __shfl_xor(x, m);
+        return x;
+    } else {
+        double z[16], b[8], be[8], c[4], ce[4], d[2], de[2], e[1], ee[1];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) z[k] = 0.0;
+        halve_c<M1>(a, z, b, be, 8, lane & M1);
+        halve_c<M2>(b, be, c, ce, 4, lane & M2);
+        halve_c<M3>(c, ce, d, de, 2, lane & M3);
+        halve_c<M4>(d, de, e, ee, 1, lane & M4);
+        double x = e[0], xe = ee[0];
+#pragma unroll
+        for (int m = M4 / 2; m >= 1; m >>= 1) {
+            double s2, err;
+            two_sum(x, __shfl_xor(x, m), s2, err);
+            xe = xe + __shfl_xor(xe, m) + err;
+            x = s2;
+        }
+        return x + xe;
+    }
+}
+
+// x rows staged per wave: i0-2 .. i0+R+1, L directions each
+template <typename T, int R>
+struct Slab {
+    static constexpr int L = kWave / R;
+    static constexpr int NS = R + 4;
+    T v[2][kNVar][NS * L];   // two z levels (by parity) of all 8 fields
+};
+
+// One wavefront's work: task = 4 * lblock + wave-in-block.
+template <typename T, int R, int FPD, int STAGE, bool COMP>
+__device__ __forceinline__ void stage_body(const StageArgs<T, FPD>& a, int lblock, int wib, Slab<T, R>& slab) {
+    constexpr int L = kWave / R;   // directions per row
+    constexpr int NH = 4 / R;      // halo loads per field
+    auto& xs = slab.v;
+
     const int lane = threadIdx.x & 63;
-    const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int task = xcd_remap(blockIdx.x, gridDim.x) * 4 + wib;
+    const int task = lblock * 4 + wib;
     if (task >= a.ntask) return;
 
     const int nx = a.g.nx, nz = a.g.nz, nch = a.g.nch, nv = a.g.nv;
-    const unsigned rowb = (unsigned)a.g.nvp * sizeof(T);   // bytes per site row
-    const int i = task % nx;
-    const int rest = task / nx;
-    const int ch = rest % nch;
-    const int zs = rest / nch;
-    const int v = ch * kWave + lane;
-    const bool act = v < nv;
-    const int vl = act ? v : nv - 1;
-    const unsigned vo = (unsigned)vl * sizeof(T);
+    const unsigned rowb = (unsigned)a.g.nvp * sizeof(T);  // bytes per site row
+    // direction chunk fastest: the waves of one block (and of neighbouring
+    // blocks, which the XCD remap keeps together) read adjacent 128-byte
+    // segments of the same site rows, and the x-row quads next to each other
+    // run at the same time (their halo reads hit in L2)
+    const int ch = task % nch;
+    const int rest = task / nch;
+    const int qd = rest % a.nquads;
+    const int zs = rest / a.nquads;
+    const int i0 = qd * R;
+    const int g = lane / L, dl = lane % L;
+    const int v = ch * L + dl;
+    const int r = i0 + g;
+    const bool row_ok = r < nx;
+    const bool act = row_ok && v < nv;
+    const int vl = min(v, nv - 1);
+    const int rw = wrapi(r, nx);
+
+    // lowest referenced row = scalar base; per-lane row deltas (>= 0) in the
+    // vector offsets
+    int rmin = wrapi(i0 - 2, nx);
+    rmin = min(rmin, wrapi(i0 - 1, nx));
+    for (int q = 0; q < R + 2; ++q) rmin = min(rmin, wrapi(i0 + q, nx));
+    const unsigned rbase = (unsigned)rmin * nz;
+    const unsigned fieldb = (unsigned)nz * rowb;  // bytes per x-row of one field
+    const unsigned vo = (unsigned)(rw - rmin) * fieldb + (unsigned)vl * sizeof(T);
+    unsigned vh[NH];
+    int slot_h[NH];
+#pragma unroll
+    for (int h = 0; h < NH; ++h) {
+        const int k = h * R + g;   // halo index 0..3 -> rows i0-2, i0-1, i0+R, i0+R+1
+        const int hr = wrapi(k < 2 ? i0 - 2 + k : i0 + R + (k - 2), nx);
+        vh[h] = (unsigned)(hr - rmin) * fieldb + (unsigned)vl * sizeof(T);
+        slot_h[h] = (k < 2 ? k : R + k) * L + dl;
+    }
+    const int slot_c = (g + 2) * L + dl;
 
     // per-direction constants (nuosc_class.cpp:119-121, 180-181)
     const double vxd = a.g.vx[vl], vyd = a.g.vy[vl], vzd = a.g.vz[vl];
@@ -253,58 +374,51 @@ __global__ __launch_bounds__(256) void k_stage(const StageArgs<T, FPD> a) {
     const T mu2 = (T)a.mu2, mu = (T)a.mu, p2st = (T)a.p2st, p2ct = (T)a.p2ct;
     const T pmo = (T)a.pmo, ct2 = (T)a.ct2, st = (T)a.st;
     const T cw = (T)a.cw, cy = (T)a.cy;
+    const unsigned vom = (unsigned)(rw - rmin) * (unsigned)nz * (kMomStride * 8);
 
-    // The five x rows i-2..i+2 of the stencil (periodic wrap).  Loads use the
-    // lowest of them as the scalar base and a per-row byte delta (>= 0) in
-    // the vector offset, so one scalar offset per field group serves all
-    // five rows.
-    int rows[5];
-#pragma unroll
-    for (int k = 0; k < 5; ++k) rows[k] = wrapi(i - 2 + k, nx);
-    int rmin = rows[0];
-#pragma unroll
-    for (int k = 1; k < 5; ++k) rmin = min(rmin, rows[k]);
-    unsigned vrow[5];
-#pragma unroll
-    for (int k = 0; k < 5; ++k) vrow[k] = (unsigned)(rows[k] - rmin) * (unsigned)nz * rowb + vo;
-    const unsigned r0 = (unsigned)i * nz;
-    const unsigned rbase = (unsigned)rmin * nz;
-
-    cdouble* mtot = (cdouble*)a.mtot;
     const int j0 = zs * a.zlen;
     const int j1 = min(nz, j0 + a.zlen);
 
-    // z-window: win[f][0..4] = field f at j-2 .. j+2 of row i
-    T win[kNVar][5];
+    T win[kNVar][5];   // own row, z = j-2 .. j+2
+    T hreg[NH][kNVar]; // halo rows loaded for z = j+2, staged into LDS one iteration later
 #pragma unroll
     for (int qz = 0; qz < 4; ++qz) {
         const unsigned so = (rbase + (unsigned)wrapi(j0 - 2 + qz, nz)) * rowb;
 #pragma unroll
-        for (int f = 0; f < kNVar; ++f) win[f][qz] = a.yin.ld(a.yin.base, f, vrow[2], so);
+        for (int f = 0; f < kNVar; ++f) win[f][qz] = a.yin.ld(a.yin.base, f, vo, so);
     }
-    // moments of the first site; later ones are fetched one site ahead
-    double M[kNMom];
+    {
+        const unsigned so = (rbase + (unsigned)wrapi(j0, nz)) * rowb;
 #pragma unroll
-    for (int s = 0; s < kNMom; ++s) M[s] = mtot[(long long)(r0 + j0) * kMomStride + s];
+        for (int h = 0; h < NH; ++h)
+#pragma unroll
+            for (int f = 0; f < kNVar; ++f) xs[j0 & 1][f][slot_h[h]] = a.yin.ld(a.yin.base, f, vh[h], so);
+        const unsigned s1 = (rbase + (unsigned)wrapi(j0 + 1, nz)) * rowb;
+#pragma unroll
+        for (int h = 0; h < NH; ++h)
+#pragma unroll
+            for (int f = 0; f < kNVar; ++f) hreg[h][f] = a.yin.ld(a.yin.base, f, vh[h], s1);
+    }
 
     for (int j = j0; j < j1; ++j) {
-        const unsigned site = r0 + j;
-        const unsigned so = site * rowb;        // own row, own offsets
-        const unsigned sx = (rbase + j) * rowb;  // x rows, with vrow[k]
-        // all loads of this iteration first
+        const unsigned so = (rbase + j) * rowb;
+        const int lv = j & 1;
+        wave_sync();   // the previous iteration's LDS reads stay before these writes
+        // halo rows of z = j+1 (loaded one iteration ago) into the other level
+#pragma unroll
+        for (int h = 0; h < NH; ++h)
+#pragma unroll
+            for (int f = 0; f < kNVar; ++f) xs[lv ^ 1][f][slot_h[h]] = hreg[h][f];
+        // ---- all loads of the iteration
         const unsigned long long by = a.yin.fresh_base();
         {
             const unsigned s2 = (rbase + (unsigned)wrapi(j + 2, nz)) * rowb;
 #pragma unroll
-            for (int f = 0; f < kNVar; ++f) win[f][4] = a.yin.ld(by, f, vrow[2], s2);
-        }
-        T xm2[kNVar], xm1[kNVar], xp1[kNVar], xp2[kNVar];
+            for (int f = 0; f < kNVar; ++f) win[f][4] = a.yin.ld(by, f, vo, s2);
 #pragma unroll
-        for (int f = 0; f < kNVar; ++f) {
-            xm2[f] = a.yin.ld(by, f, vrow[0], sx);
-            xm1[f] = a.yin.ld(by, f, vrow[1], sx);
-            xp1[f] = a.yin.ld(by, f, vrow[3], sx);
-            xp2[f] = a.yin.ld(by, f, vrow[4], sx);
+            for (int h = 0; h < NH; ++h)
+#pragma unroll
+                for (int f = 0; f < kNVar; ++f) hreg[h][f] = a.yin.ld(by, f, vh[h], s2);
         }
         T uc[kNVar], wc[kNVar];
         if constexpr (STAGE == 2 || STAGE == 3) {
@@ -317,15 +431,29 @@ __global__ __launch_bounds__(256) void k_stage(const StageArgs<T, FPD> a) {
 #pragma unroll
             for (int f = 0; f < kNVar; ++f) wc[f] = a.w.ld(bw, f, vo, so);
         }
-        // projections of the site's velocity moments onto direction v
-        // (INP, nuosc_class.cpp:187)
+        // velocity moments of the stage input at this lane's site
+        double M[kNMom];
+        {
+            unsigned long long bm = a.mtot;
+            asm volatile("" : "+s"(bm));
+            const __amdgpu_buffer_rsrc_t rm =
+                __builtin_amdgcn_make_buffer_rsrc((void*)bm, 0, (int)a.mtot_bytes, kRsrcWord3);
+            const unsigned sm = (rbase + j) * (kMomStride * 8);
+#pragma unroll
+            for (int s = 0; s < kNMom; s += 2) {
+                const u32x4 x = __builtin_amdgcn_raw_buffer_load_b128(rm, vom + s * 8, sm, 0);
+                M[s] = __builtin_bit_cast(double, u32x2{x[0], x[1]});
+                M[s + 1] = __builtin_bit_cast(double, u32x2{x[2], x[3]});
+            }
+        }
+        // ---- own rows of z = j into this level; halo rows of z = j are there
+#pragma unroll
+        for (int f = 0; f < kNVar; ++f) xs[lv][f][slot_c] = win[f][2];
+        wave_sync();   // other lanes' writes are visible to the reads below
+        // projections onto direction v (INP, nuosc_class.cpp:187)
         const T A = (T)(M[0] - M[1] * vxd - M[2] * vyd - M[3] * vzd);
         const T B = (T)(M[4] - M[5] * vxd - M[6] * vyd - M[7] * vzd);
         const T C = (T)(M[8] - M[9] * vxd - M[10] * vyd - M[11] * vzd);
-        if (j + 1 < j1) {
-#pragma unroll
-            for (int s = 0; s < kNMom; ++s) M[s] = mtot[(long long)(site + 1) * kMomStride + s];
-        }
 
         const T ee = win[0][2], mm = win[1][2], emr = win[2][2], emi = win[3][2];
         const T bee = win[4][2], bmm = win[5][2], bemr = win[6][2], bemi = win[7][2];
@@ -343,12 +471,14 @@ __global__ __launch_bounds__(256) void k_stage(const StageArgs<T, FPD> a) {
         T k[kNVar];
 #pragma unroll
         for (int f = 0; f < kNVar; ++f) {
+            const T xm2 = xs[lv][f][slot_c - 2 * L], xm1 = xs[lv][f][slot_c - L];
+            const T xp1 = xs[lv][f][slot_c + L], xp2 = xs[lv][f][slot_c + 2 * L];
             const T zm2 = win[f][0], zm1 = win[f][1], c0 = win[f][2], zp1 = win[f][3], zp2 = win[f][4];
             // advection (nuosc_class.cpp:182-183)
-            const T adv = fz * ((zm2 - zp2) - (T)8.0 * (zm1 - zp1)) + fx * ((xm2[f] - xp2[f]) - (T)8.0 * (xm1[f] - xp1[f]));
+            const T adv = fz * ((zm2 - zp2) - (T)8.0 * (zm1 - zp1)) + fx * ((xm2 - xp2) - (T)8.0 * (xm1 - xp1));
             // Kreiss-Oliger (nuosc_class.cpp:122-123)
             const T ko = kz * (zm2 + zp2 - (T)4 * (zm1 + zp1) + (T)6 * c0) +
-                         kx * (xm2[f] + xp2[f] - (T)4 * (xm1[f] + xp1[f]) + (T)6 * c0);
+                         kx * (xm2 + xp2 - (T)4 * (xm1 + xp1) + (T)6 * c0);
             k[f] = loc[f] + adv + ko + vac[f];
         }
 
@@ -379,10 +509,11 @@ __global__ __launch_bounds__(256) void k_stage(const StageArgs<T, FPD> a) {
         if constexpr (STAGE != 0) {
             double t[16];
             moment_terms<T>(yo, q, t);
-            const double tot = wave_sum16<COMP>(t, lane);
-            const int m = lane >> 2;
-            if ((lane & 3) == 0 && m < kNMom)
-                a.mpart[((long long)site * nch + ch) * kMomStride + m] = tot;
+            const double tot = group_sum16<L, COMP>(t, lane);
+            constexpr int sh = (L == 16) ? 0 : (L == 32 ? 1 : 2);
+            const int m = dl >> sh;
+            if (row_ok && (dl & ((1 << sh) - 1)) == 0 && m < kNMom)
+                a.mpart[((long long)(rw * nz + j) * nch + ch) * kMomRec + m] = tot;
         }
 #pragma unroll
         for (int f = 0; f < kNVar; ++f) {
@@ -394,17 +525,43 @@ __global__ __launch_bounds__(256) void k_stage(const StageArgs<T, FPD> a) {
     }
 }
 
+template <typename T, int R, int FPD, int STAGE, bool COMP>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_stage(const StageArgs<T, FPD> a) {
+    __shared__ Slab<T, R> slab[4];
+    const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    stage_body<T, R, FPD, STAGE, COMP>(a, xcd_remap(blockIdx.x, gridDim.x), wib, slab[wib]);
+}
+
+// Many independent runs in one launch: logical block -> (run, block of run)
+// from bmap; each run's arguments from the runs table (read with scalar loads).
+template <typename T, int R, int FPD, int STAGE, bool COMP>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_stage_batch(
+    const StageArgs<T, FPD>* __restrict__ runs, const int* __restrict__ bmap) {
+    __shared__ Slab<T, R> slab[4];
+    const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lb = xcd_remap(blockIdx.x, gridDim.x);
+    const __attribute__((address_space(4))) int* cm = (const __attribute__((address_space(4))) int*)bmap;
+    const int run = cm[2 * lb], blk = cm[2 * lb + 1];
+    const StageArgs<T, FPD> a = runs[run];   // uniform, read-only: scalar loads
+    stage_body<T, R, FPD, STAGE, COMP>(a, blk, wib, slab[wib]);
+}
+
+int rows_per_wave(int esize) { return esize == 8 ? 4 : 2; }
+
 StagePlan plan_stage(const Geometry& g, int num_cus) {
     // enough waves to give every SIMD several (>= 12 per CU), segments long
-    // enough that the 4-row window warm-up stays a small overhead
+    // enough that the window warm-up stays a small overhead
     StagePlan p{};
-    const long long rows = (long long)g.nx * g.nch;
+    const int R = kWave / g.lpr;
+    const long long quads = (g.nx + R - 1) / R;
+    const long long rows = quads * g.nch;
     const long long want = (long long)num_cus * 48;
     int zlen = g.nz;
     while (zlen > 8 && rows * ((g.nz + zlen - 1) / zlen) < want) zlen = (zlen + 1) / 2;
     if (zlen > 64 && rows * ((g.nz + 63) / 64) >= want) zlen = 64;
     p.zlen = zlen;
     p.nzs = (g.nz + zlen - 1) / zlen;
+    p.nquads = (int)quads;
     p.ntask = (int)(rows * p.nzs);
     p.nblocks = (p.ntask + 3) / 4;
     return p;
@@ -433,16 +590,17 @@ static Fields<T, FPD> fields_of(const Geometry& g, const T* p) {
 }
 
 template <typename T, int FPD>
-static hipError_t launch_stage_fpd(int stage, const Geometry& g, const Physics& ph, const StagePlan& pl,
-                                   const T* yin, const T* u, T* w, T* yout, const double* mtot, double* mpart,
-                                   double cw, double cy, hipStream_t s) {
+static StageArgs<T, FPD> make_stage_args(const Geometry& g, const Physics& ph, const StagePlan& pl, const T* yin,
+                                         const T* u, T* w, T* yout, const double* mtot, double* mpart, double cw,
+                                         double cy) {
     StageArgs<T, FPD> a;
     a.g = g;
     a.yin = fields_of<T, FPD>(g, yin);
     a.u = fields_of<T, FPD>(g, u ? u : yin);
     a.w = fields_of<T, FPD>(g, w ? w : yout);
     a.yout = fields_of<T, FPD>(g, yout);
-    a.mtot = mtot;
+    a.mtot = (unsigned long long)mtot;
+    a.mtot_bytes = (unsigned)((long long)g.nx * g.nz * kMomStride * sizeof(double));
     a.mpart = mpart;
     a.cw = cw;
     a.cy = cy;
@@ -455,14 +613,25 @@ static hipError_t launch_stage_fpd(int stage, const Geometry& g, const Physics& 
     a.st = ph.st;
     a.zlen = pl.zlen;
     a.ntask = pl.ntask;
+    a.nquads = pl.nquads;
+    return a;
+}
+
+template <typename T, int FPD>
+static hipError_t launch_stage_fpd(int stage, const Geometry& g, const Physics& ph, const StagePlan& pl,
+                                   const T* yin, const T* u, T* w, T* yout, const double* mtot, double* mpart,
+                                   double cw, double cy, hipStream_t s) {
+    const StageArgs<T, FPD> a = make_stage_args<T, FPD>(g, ph, pl, yin, u, w, yout, mtot, mpart, cw, cy);
     constexpr bool COMP = sizeof(T) == 4;
+    constexpr int R = sizeof(T) == 8 ? 4 : 2;
+    if (g.lpr != kWave / R) return hipErrorInvalidValue;
     dim3 grid(pl.nblocks), block(256);
     switch (stage) {
-        case 0: hipLaunchKernelGGL((k_stage<T, FPD, 0, COMP>), grid, block, 0, s, a); break;
-        case 1: hipLaunchKernelGGL((k_stage<T, FPD, 1, COMP>), grid, block, 0, s, a); break;
-        case 2: hipLaunchKernelGGL((k_stage<T, FPD, 2, COMP>), grid, block, 0, s, a); break;
-        case 3: hipLaunchKernelGGL((k_stage<T, FPD, 3, COMP>), grid, block, 0, s, a); break;
-        case 4: hipLaunchKernelGGL((k_stage<T, FPD, 4, COMP>), grid, block, 0, s, a); break;
+        case 0: hipLaunchKernelGGL((k_stage<T, R, FPD, 0, COMP>), grid, block, 0, s, a); break;
+        case 1: hipLaunchKernelGGL((k_stage<T, R, FPD, 1, COMP>), grid, block, 0, s, a); break;
+        case 2: hipLaunchKernelGGL((k_stage<T, R, FPD, 2, COMP>), grid, block, 0, s, a); break;
+        case 3: hipLaunchKernelGGL((k_stage<T, R, FPD, 3, COMP>), grid, block, 0, s, a); break;
+        case 4: hipLaunchKernelGGL((k_stage<T, R, FPD, 4, COMP>), grid, block, 0, s, a); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -480,32 +649,44 @@ hipError_t launch_stage(int stage, const Geometry& g, const Physics& ph, const S
     }
 }
 
+struct TotArgs {
+    const double* part;
+    double* tot;
+    int nch;
+    long long nsites;
+};
+
+template <bool COMP>
+__device__ __forceinline__ void totals_body(const double* __restrict__ part, int nch, long long nsites,
+                                            double* __restrict__ tot, long long n) {
+    if (n >= nsites * kMomStride) return;
+    const long long site = n / kMomStride;
+    const int s = (int)(n % kMomStride);
+    if (s >= kNMom) { tot[n] = 0.0; return; }
+    const double* rec = part + site * nch * kMomRec + s;
+    double sum = rec[0];
+    if constexpr (!COMP) {
+        for (int c = 1; c < nch; ++c) sum += rec[c * kMomRec];
+        tot[n] = sum;
+    } else {
+        double err = 0.0;
+        for (int c = 1; c < nch; ++c) {
+            double s2, e;
+            two_sum(sum, rec[c * kMomRec], s2, e);
+            sum = s2;
+            err += e;
+        }
+        tot[n] = sum + err;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // per-site moment totals: sum of the nch chunk records in chunk order
 // ---------------------------------------------------------------------------
 template <bool COMP>
 __global__ __launch_bounds__(256) void k_moment_totals(const double* __restrict__ part, int nch,
                                                        long long nsites, double* __restrict__ tot) {
-    const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= nsites * kMomStride) return;
-    const long long site = n / kMomStride;
-    const int s = (int)(n % kMomStride);
-    if (s >= kNMom) { tot[n] = 0.0; return; }
-    const double* rec = part + site * nch * kMomStride + s;
-    double sum = rec[0];
-    if constexpr (!COMP) {
-        for (int c = 1; c < nch; ++c) sum += rec[c * kMomStride];
-        tot[n] = sum;
-    } else {
-        double err = 0.0;
-        for (int c = 1; c < nch; ++c) {
-            double s2, e;
-            two_sum(sum, rec[c * kMomStride], s2, e);
-            sum = s2;
-            err += e;
-        }
-        tot[n] = sum + err;
-    }
+    totals_body<COMP>(part, nch, nsites, tot, (long long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 hipError_t launch_moment_totals(const Geometry& g, bool compensated, const double* part, double* tot,
@@ -519,37 +700,186 @@ hipError_t launch_moment_totals(const Geometry& g, bool compensated, const doubl
 }
 
 // ---------------------------------------------------------------------------
-// stand-alone moment pass (after init / upload): one wave per (site, chunk)
+// batched runs: one launch per RK4 stage (and one per moment-total pass) for
+// every member run; the runs keep their own buffers, plans and physics
+// ---------------------------------------------------------------------------
+template <bool COMP>
+__global__ __launch_bounds__(256) void k_moment_totals_batch(const TotArgs* __restrict__ runs,
+                                                             const int* __restrict__ bmap) {
+    const __attribute__((address_space(4))) int* cm = (const __attribute__((address_space(4))) int*)bmap;
+    const int run = cm[2 * blockIdx.x], blk = cm[2 * blockIdx.x + 1];
+    const TotArgs ta = runs[run];
+    totals_body<COMP>(ta.part, ta.nch, ta.nsites, ta.tot, (long long)blk * 256 + threadIdx.x);
+}
+
+template <typename T, int FPD>
+static hipError_t batch_build_t(BatchTables& t, const std::vector<BatchMember>& ms, hipStream_t s) {
+    const int n = (int)ms.size();
+    std::vector<int> bmap, tmap;
+    for (int r = 0; r < n; ++r)
+        for (int b = 0; b < ms[r].pl.nblocks; ++b) {
+            bmap.push_back(r);
+            bmap.push_back(b);
+        }
+    for (int r = 0; r < n; ++r) {
+        const long long ent = (long long)ms[r].g.nx * ms[r].g.nz * kMomStride;
+        const int nb = (int)((ent + 255) / 256);
+        for (int b = 0; b < nb; ++b) {
+            tmap.push_back(r);
+            tmap.push_back(b);
+        }
+    }
+    t.nblocks = (int)bmap.size() / 2;
+    t.tblocks = (int)tmap.size() / 2;
+    hipError_t e;
+    if ((e = hipMalloc((void**)&t.bmap, bmap.size() * sizeof(int))) != hipSuccess) return e;
+    if ((e = hipMalloc((void**)&t.tmap, tmap.size() * sizeof(int))) != hipSuccess) return e;
+    if ((e = hipMemcpyAsync(t.bmap, bmap.data(), bmap.size() * sizeof(int), hipMemcpyHostToDevice, s)) != hipSuccess)
+        return e;
+    if ((e = hipMemcpyAsync(t.tmap, tmap.data(), tmap.size() * sizeof(int), hipMemcpyHostToDevice, s)) != hipSuccess)
+        return e;
+    for (int st = 1; st <= 4; ++st) {
+        std::vector<StageArgs<T, FPD>> args(n);
+        std::vector<TotArgs> tots(n);
+        for (int r = 0; r < n; ++r) {
+            const BatchMember& m = ms[r];
+            const double dt = m.dt, c6 = 1.0 / 6.0 * dt, c3 = 1.0 / 3.0 * dt, c2 = 0.5 * dt;
+            T *u = (T*)m.u, *w = (T*)m.w, *ya = (T*)m.ya, *yb = (T*)m.yb;
+            const T* yin;
+            const T* uu;
+            T* yout;
+            const double* mt;
+            double* mo;
+            double cw, cy;
+            switch (st) {
+                case 1: yin = u; uu = u; yout = ya; mt = m.mu; mo = m.ma; cw = c6; cy = c2; break;
+                case 2: yin = ya; uu = u; yout = yb; mt = m.ma; mo = m.mb; cw = c3; cy = c2; break;
+                case 3: yin = yb; uu = u; yout = ya; mt = m.mb; mo = m.ma; cw = c3; cy = dt; break;
+                default: yin = ya; uu = ya; yout = u; mt = m.ma; mo = m.mu; cw = c6; cy = 0.0; break;
+            }
+            args[r] = make_stage_args<T, FPD>(m.g, m.ph, m.pl, yin, uu, w, yout, mt, m.mpart, cw, cy);
+            tots[r] = TotArgs{m.mpart, mo, m.g.nch, (long long)m.g.nx * m.g.nz};
+        }
+        if ((e = hipMalloc(&t.args[st - 1], n * sizeof(StageArgs<T, FPD>))) != hipSuccess) return e;
+        if ((e = hipMalloc(&t.tot[st - 1], n * sizeof(TotArgs))) != hipSuccess) return e;
+        if ((e = hipMemcpyAsync(t.args[st - 1], args.data(), n * sizeof(StageArgs<T, FPD>), hipMemcpyHostToDevice,
+                                s)) != hipSuccess)
+            return e;
+        if ((e = hipMemcpyAsync(t.tot[st - 1], tots.data(), n * sizeof(TotArgs), hipMemcpyHostToDevice, s)) !=
+            hipSuccess)
+            return e;
+    }
+    return hipStreamSynchronize(s);
+}
+
+hipError_t batch_build(BatchTables& t, const std::vector<BatchMember>& ms, int esize, hipStream_t s) {
+    batch_free(t);
+    t.esize = esize;
+    if (ms.empty()) return hipSuccess;
+    int fpd = 4;
+    for (const auto& m : ms) {
+        if (!buffer_addressable(m.g, esize)) return hipErrorInvalidValue;
+        fpd = std::min(fpd, fields_per_descriptor(m.g, esize));
+        if (m.g.lpr != kWave / rows_per_wave(esize)) return hipErrorInvalidValue;
+    }
+    t.fpd = fpd;
+    if (esize == 8) {
+        if (fpd == 4) return batch_build_t<double, 4>(t, ms, s);
+        if (fpd == 2) return batch_build_t<double, 2>(t, ms, s);
+        return batch_build_t<double, 1>(t, ms, s);
+    }
+    if (fpd == 4) return batch_build_t<float, 4>(t, ms, s);
+    if (fpd == 2) return batch_build_t<float, 2>(t, ms, s);
+    return batch_build_t<float, 1>(t, ms, s);
+}
+
+void batch_free(BatchTables& t) {
+    for (int k = 0; k < 4; ++k) {
+        if (t.args[k]) (void)hipFree(t.args[k]);
+        if (t.tot[k]) (void)hipFree(t.tot[k]);
+        t.args[k] = t.tot[k] = nullptr;
+    }
+    if (t.bmap) (void)hipFree(t.bmap);
+    if (t.tmap) (void)hipFree(t.tmap);
+    t.bmap = t.tmap = nullptr;
+    t.nblocks = t.tblocks = 0;
+}
+
+template <typename T, int FPD>
+static hipError_t batch_step_t(const BatchTables& t, hipStream_t s) {
+    constexpr int R = sizeof(T) == 8 ? 4 : 2;
+    constexpr bool COMP = sizeof(T) == 4;
+    const dim3 grid(t.nblocks), block(256), tgrid(t.tblocks);
+    for (int st = 1; st <= 4; ++st) {
+        const StageArgs<T, FPD>* a = (const StageArgs<T, FPD>*)t.args[st - 1];
+        switch (st) {
+            case 1: hipLaunchKernelGGL((k_stage_batch<T, R, FPD, 1, COMP>), grid, block, 0, s, a, t.bmap); break;
+            case 2: hipLaunchKernelGGL((k_stage_batch<T, R, FPD, 2, COMP>), grid, block, 0, s, a, t.bmap); break;
+            case 3: hipLaunchKernelGGL((k_stage_batch<T, R, FPD, 3, COMP>), grid, block, 0, s, a, t.bmap); break;
+            default: hipLaunchKernelGGL((k_stage_batch<T, R, FPD, 4, COMP>), grid, block, 0, s, a, t.bmap); break;
+        }
+        hipLaunchKernelGGL((k_moment_totals_batch<COMP>), tgrid, block, 0, s, (const TotArgs*)t.tot[st - 1], t.tmap);
+    }
+    return hipGetLastError();
+}
+
+hipError_t batch_step(const BatchTables& t, hipStream_t s) {
+    if (t.nblocks == 0) return hipSuccess;
+    if (t.esize == 8) {
+        if (t.fpd == 4) return batch_step_t<double, 4>(t, s);
+        if (t.fpd == 2) return batch_step_t<double, 2>(t, s);
+        return batch_step_t<double, 1>(t, s);
+    }
+    if (t.fpd == 4) return batch_step_t<float, 4>(t, s);
+    if (t.fpd == 2) return batch_step_t<float, 2>(t, s);
+    return batch_step_t<float, 1>(t, s);
+}
+
+// ---------------------------------------------------------------------------
+// stand-alone partial moments of a state (after init / upload): one thread
+// per (site, chunk), directions of the chunk summed in order
 // ---------------------------------------------------------------------------
 template <typename T, bool COMP>
 __global__ __launch_bounds__(256) void k_moments(const Geometry g, const T* __restrict__ y,
-                                                 double* __restrict__ mout, long long ntask) {
-    const int lane = threadIdx.x & 63;
-    const long long task = (long long)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    if (task >= ntask) return;
-    const long long site = task / g.nch;
-    const int ch = (int)(task % g.nch);
-    const int v = ch * kWave + lane;
-    const bool act = v < g.nv;
-    const int vl = act ? v : g.nv - 1;
-    const double vwd = act ? g.vw[vl] : 0.0;
-    const double q[4] = {vwd, g.vx[vl] * vwd, g.vy[vl] * vwd, g.vz[vl] * vwd};
-    T yv[kNVar];
+                                                 double* __restrict__ mpart) {
+    const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long nrec = (long long)g.nx * g.nz * g.nch;
+    if (n >= nrec) return;
+    const long long site = n / g.nch;
+    const int ch = (int)(n % g.nch);
+    double S[kNMom], E[kNMom];
 #pragma unroll
-    for (int f = 0; f < kNVar; ++f) yv[f] = y[f * g.fs + site * g.nvp + vl];
-    double t[16];
-    moment_terms<T>(yv, q, t);
-    const double tot = wave_sum16<COMP>(t, lane);
-    const int m = lane >> 2;
-    if ((lane & 3) == 0 && m < kNMom) mout[(site * g.nch + ch) * kMomStride + m] = tot;
+    for (int s = 0; s < kNMom; ++s) { S[s] = 0.0; E[s] = 0.0; }
+    const int v1 = min(g.nv, (ch + 1) * g.lpr);
+    for (int v = ch * g.lpr; v < v1; ++v) {
+        T yv[kNVar];
+#pragma unroll
+        for (int f = 0; f < kNVar; ++f) yv[f] = y[f * g.fs + site * g.nvp + v];
+        const double vw = g.vw[v];
+        const double q[4] = {vw, g.vx[v] * vw, g.vy[v] * vw, g.vz[v] * vw};
+        double t[16];
+        moment_terms<T>(yv, q, t);
+#pragma unroll
+        for (int s = 0; s < kNMom; ++s) {
+            if constexpr (COMP) {
+                double s2, err;
+                two_sum(S[s], t[s], s2, err);
+                S[s] = s2;
+                E[s] += err;
+            } else {
+                S[s] += t[s];
+            }
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < kNMom; ++s) mpart[n * kMomRec + s] = S[s] + E[s];
 }
 
 template <typename T>
-hipError_t launch_moments(const Geometry& g, const T* y, double* mout, hipStream_t s) {
-    const long long ntask = (long long)g.nx * g.nz * g.nch;
-    const long long nb = (ntask + 3) / 4;
-    hipLaunchKernelGGL((k_moments<T, sizeof(T) == 4>), dim3((unsigned)nb), dim3(256), 0, s, g, y, mout,
-                       ntask);
+hipError_t launch_moments(const Geometry& g, const T* y, double* mpart, hipStream_t s) {
+    const long long nrec = (long long)g.nx * g.nz * g.nch;
+    hipLaunchKernelGGL((k_moments<T, sizeof(T) == 4>), dim3((unsigned)((nrec + 255) / 256)), dim3(256), 0, s, g,
+                       y, mpart);
     return hipGetLastError();
 }
 

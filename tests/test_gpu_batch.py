@@ -1,0 +1,76 @@
+"""Batched runs: members with different grids, direction counts and physics
+advance together (one launch per RK4 stage for all) and every member's state
+is bitwise the one it reaches stepped alone; members can leave and join."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+RUNS = [
+    dict(nv=2, x0=-0.8, x1=0.8, z0=-1.2, z1=1.2, dx=0.1, dz=0.1, CFL=0.4, ko=0.1, mu=1.0, pmo=0.0),
+    dict(nv=4, x0=-1.6, x1=1.6, z0=-0.8, z1=0.8, dx=0.1, dz=0.1, CFL=0.4, ko=0.05, mu=0.7, pmo=1.0),
+    dict(nv=3, x0=-0.4, x1=0.4, z0=-0.4, z1=0.4, dx=0.05, dz=0.05, CFL=0.3, ko=0.1, mu=1.5, pmo=0.0),
+    dict(nv=1, x0=0.0, x1=0.1, z0=-3.2, z1=3.2, dx=0.1, dz=0.1, CFL=0.4, ko=0.1, mu=1.0, pmo=0.0),
+]
+
+
+def make(N, cfg, seed, precision):
+    g = N.NuOsc(precision=precision, **cfg)
+    g.fillInitValue(ipt=5, alpha=0.85, eps0=1e-3, sigma=1.0, lnue=0.6, lnueb=0.5, seed=seed)
+    return g
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_batch_equals_alone(nuosc_mod, precision):
+    N = nuosc_mod
+    alone = [make(N, c, 100 + k, precision) for k, c in enumerate(RUNS)]
+    mixed = [make(N, c, 100 + k, precision) for k, c in enumerate(RUNS)]
+    b = N.Batch(precision=precision)
+    for g in mixed[:3]:
+        b.add(g)
+    assert len(b) == 3
+    b.step(7)
+    for g in alone[:3]:
+        g.step_rk4(7)
+    for a, m in zip(alone[:3], mixed[:3]):
+        assert np.array_equal(a.get_state(), m.get_state())
+        assert a.phy_time == m.phy_time
+    # membership changes: one leaves, one joins
+    b.remove(mixed[1])
+    b.add(mixed[3])
+    b.step(5)
+    for k in (0, 2):
+        alone[k].step_rk4(5)
+    alone[3].step_rk4(5)
+    for k in (0, 2, 3):
+        assert np.array_equal(alone[k].get_state(), mixed[k].get_state()), k
+        a1, a2 = alone[k].analysis(), mixed[k].analysis()
+        assert a1.surv == a2.surv and a1.maxdP == a2.maxdP
+    # the member that left keeps working alone
+    mixed[1].step_rk4(5)
+    alone[1].step_rk4(5)
+    assert np.array_equal(alone[1].get_state(), mixed[1].get_state())
+    b.close()
+
+
+def test_batch_rejects_mismatch(nuosc_mod):
+    N = nuosc_mod
+    b = N.Batch(precision=64)
+    g32 = make(N, RUNS[0], 1, 32)
+    with pytest.raises(N.NuOscError):
+        b.add(g32)
+    g = make(N, RUNS[0], 1, 64)
+    b.add(g)
+    with pytest.raises(N.NuOscError):
+        b.add(g)
+
+
+def test_estimate_matches_allocation(nuosc_mod):
+    N = nuosc_mod
+    for c in RUNS:
+        for p in (64, 32):
+            g = N.NuOsc(precision=p, **c)
+            g.fillInitValue(ipt=0, alpha=0.9, eps0=1e-6, sigma=1.0, lnue=0.6, lnueb=0.5)
+            est = N.estimate_device_bytes(precision=p, **c)
+            held = g.device_bytes()
+            assert est >= held * 0.95 and est <= held * 1.5 + (1 << 20), (est, held)

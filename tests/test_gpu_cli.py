@@ -1,0 +1,88 @@
+"""The ./nuosc drop-in (nuosc_amd/bin/nuosc) on the GPU: analysis.dat against
+the oracle's lines, the reference's stdout lines, P3 snapshot files in the
+reference's binary format (nuosc_snapshot.cpp:82-122), renormalisation, and
+the unknown-flag behaviour (nuosc.cpp:135-137)."""
+import os
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(ROOT, "nuosc_amd", "bin", "nuosc")
+
+ARGS = ["--zmax", "1.6", "--xmax", "0.8", "--dz", "0.1", "--nv", "3", "--mu", "1", "--pmo", "1", "--ko", "0.1",
+        "--cfl", "0.4", "--ipt", "0", "--eps0", "1e-3", "--sigma", "1", "--alpha", "0.9", "--lnue", "0.6",
+        "--lnueb", "0.5", "--END_STEP", "30", "--ANA_EVERY", "10"]
+
+
+def run_cli(tmp_path, args, timeout=120):
+    return subprocess.run([CLI] + args, cwd=tmp_path, capture_output=True, text=True, timeout=timeout)
+
+
+def test_cli_analysis_dat_matches_oracle(tmp_path, oracle_mod):
+    p = run_cli(tmp_path, ARGS)
+    assert p.returncode == 0, p.stderr
+    lines = open(tmp_path / "analysis.dat").read().splitlines()
+    assert lines[0].startswith("### [ phy_time,")
+    rows = [l for l in lines[1:] if l.strip()]
+    assert len(rows) == 4
+    o = oracle_mod.OracleSim(nn=3, x0=-0.8, x1=0.8, z0=-1.6, z1=1.6, dx=0.1, dz=0.1, cfl=0.4, ko=0.1, mu=1.0,
+                             pmo=1.0)
+    o.fill_init(ipt=0, alpha=0.9, eps0=1e-3, sigma=1.0, lnue=0.6, lnueb=0.5)
+    want = []
+    for k in range(4):
+        if k:
+            o.step(10)
+        out, _ = o.analysis()
+        want.append(o.analysis_line(out))
+    for got, exp in zip(rows, want):
+        g, e = got.split(), exp.split()
+        assert len(g) == len(e) == 9
+        assert g[0] == e[0]                       # phy_time, same stream precision
+        for a, b in zip(g[1:], e[1:]):
+            a, b = float(a), float(b)
+            assert abs(a - b) <= 1e-10 * abs(b) + 1e-13, (got, exp)
+    assert rows[0].endswith(" ") and "T=" in p.stdout and "[Summ]" in p.stdout
+    assert "ns/step-grid" in p.stdout
+
+
+def test_cli_p3_snapshot_format(tmp_path):
+    p = run_cli(tmp_path, ARGS + ["--DUMP_EVERY", "15", "--snap_p3", "1"])
+    assert p.returncode == 0, p.stderr
+    files = sorted(f for f in os.listdir(tmp_path) if f.startswith("P3_"))
+    assert files == ["P3_000000.bin", "P3_000000.bin.meta", "P3_000015.bin", "P3_000030.bin"]
+    meta = open(tmp_path / "P3_000000.bin.meta").read().splitlines()
+    dt, sx, nz, sv = meta[0].split()
+    assert (int(sx), int(nz), int(sv)) == (3, 32, 3)
+    assert len(meta[3].split()) == 32 and len(meta[4].split()) == 3
+    raw = open(tmp_path / "P3_000015.bin", "rb").read()
+    t, = struct.unpack_from("<i", raw, 0)
+    ptime, = struct.unpack_from("<d", raw, 4)
+    data = np.frombuffer(raw[12:], dtype="<f8")
+    assert t == 15 and abs(ptime - 15 * 0.04) < 1e-12
+    assert data.size == 3 * 32 * 3
+    assert np.all(np.abs(data) <= 1.0 + 1e-9)
+
+
+def test_cli_renorm_matches_oracle(tmp_path, oracle_mod):
+    p = run_cli(tmp_path, ARGS + ["--renorm", "1"])
+    assert p.returncode == 0, p.stderr
+    rows = [l for l in open(tmp_path / "analysis.dat").read().splitlines()[1:] if l.strip()]
+    o = oracle_mod.OracleSim(nn=3, x0=-0.8, x1=0.8, z0=-1.6, z1=1.6, dx=0.1, dz=0.1, cfl=0.4, ko=0.1, mu=1.0,
+                             pmo=1.0, renorm=1)
+    o.fill_init(ipt=0, alpha=0.9, eps0=1e-3, sigma=1.0, lnue=0.6, lnueb=0.5)
+    o.step(30)
+    out, _ = o.analysis()
+    g = [float(x) for x in rows[-1].split()]
+    for a, b in zip(g[1:], out[1:]):
+        assert abs(a - b) <= 1e-10 * abs(b) + 1e-13
+
+
+def test_cli_unknown_flag(tmp_path):
+    p = run_cli(tmp_path, ["--bogus", "1"])
+    assert p.returncode == 0
+    assert "Unreconganized parameters --bogus!" in p.stdout
