@@ -120,6 +120,9 @@ SIGNATURES = {
     "nuosc_p3_slice": ([_vp, _ip, C.c_int, _ip, C.c_int, _dp], C.c_int),
     "nuosc_last_step_ms": ([_vp, _dp], C.c_int),
     "nuosc_profile_stages": ([_vp, C.c_int, _dp], C.c_int),
+    "nuosc_get_plan": ([_vp, _ip, _ip], C.c_int),
+    "nuosc_set_zlen": ([_vp, C.c_int], C.c_int),
+    "nuosc_autotune": ([_vp, C.c_int, _ip, _dp], C.c_int),
     "nuosc_batch_create": ([C.c_int, C.c_int, C.POINTER(_vp)], C.c_int),
     "nuosc_batch_destroy": ([_vp], C.c_int),
     "nuosc_batch_add": ([_vp, _vp], C.c_int),
@@ -289,6 +292,21 @@ class NuOsc:
         ms = np.zeros(4)
         self._check(lib().nuosc_profile_stages(self._h, int(nsteps), _ptr(ms)))
         return ms
+
+    def plan(self):
+        z, w = C.c_int(), C.c_int()
+        self._check(lib().nuosc_get_plan(self._h, C.byref(z), C.byref(w)))
+        return z.value, w.value
+
+    def set_zlen(self, zlen):
+        self._check(lib().nuosc_set_zlen(self._h, int(zlen)))
+
+    def autotune(self, trials=2):
+        """Pick the z-segment length of the stage kernel by timing it on this
+        grid (at a step boundary; the state is not modified)."""
+        z, ms = C.c_int(), C.c_double()
+        self._check(lib().nuosc_autotune(self._h, int(trials), C.byref(z), C.byref(ms)))
+        return z.value, ms.value
 
     def calRHS(self, state):
         """RHS of an interior state array (8, nx, nz, nv) -> (8, nx, nz, nv)."""

@@ -6,10 +6,14 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "nuosc_dev.h"
@@ -530,6 +534,92 @@ int nuosc_profile_stages(nuosc_sim* s, int nsteps, double* stage_ms) {
     for (auto& e : ev) hipEventDestroy(e);
     for (int k = 0; k < 4; ++k) stage_ms[k] = acc[k] / nsteps;
     return rc;
+}
+
+int nuosc_get_plan(const nuosc_sim* s, int* zlen, int* waves) {
+    if (!s) return NUOSC_EINVAL;
+    if (zlen) *zlen = s->plan.zlen;
+    if (waves) *waves = s->plan.ntask;
+    return NUOSC_OK;
+}
+
+int nuosc_set_zlen(nuosc_sim* s, int zlen) {
+    if (!s || zlen < 1) return NUOSC_EINVAL;
+    HIPCK(s, hipStreamSynchronize(s->stream));
+    s->plan = plan_stage_zlen(s->geo, zlen);
+    drop_graph(s);
+    return NUOSC_OK;
+}
+
+namespace {
+std::mutex g_tune_mu;
+std::map<std::tuple<int, int, int, int>, int> g_tune_cache;
+}  // namespace
+
+int nuosc_autotune(nuosc_sim* s, int trials, int* best_zlen, double* best_ms) {
+    if (!s || trials < 1) return NUOSC_EINVAL;
+    const auto key = std::make_tuple(s->nx, s->nz, s->nv, s->cfg.precision);
+    {
+        std::lock_guard<std::mutex> g(g_tune_mu);
+        auto it = g_tune_cache.find(key);
+        if (it != g_tune_cache.end()) {
+            int rc = nuosc_set_zlen(s, it->second);
+            if (best_zlen) *best_zlen = it->second;
+            if (best_ms) *best_ms = -1;
+            return rc;
+        }
+    }
+    if (!s->have_moments) {
+        int rc = refresh_moments(s);
+        if (rc) return rc;
+    }
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+        return fail(s, NUOSC_EHIP, "event creation failed");
+    const StagePlan keep = s->plan;
+    int best = keep.zlen;
+    float bms = 1e30f;
+    const double dt = s->dt;
+    for (int zl = 8; zl <= 256; zl *= 2) {
+        if (zl > s->nz && zl != 8) break;
+        s->plan = plan_stage_zlen(s->geo, zl);
+        float tmin = 1e30f;
+        for (int t = 0; t < trials + 1; ++t) {  // first launch warms up
+            hipError_t le;
+            (void)hipEventRecord(e0, s->stream);
+            if (is64(s))
+                le = launch_stage<double>(1, s->geo, s->ph, s->plan, (double*)s->st_u, nullptr, (double*)s->st_w,
+                                          (double*)s->st_ya, s->mom_u, s->mom_part, dt / 6, dt / 2, s->stream);
+            else
+                le = launch_stage<float>(1, s->geo, s->ph, s->plan, (float*)s->st_u, nullptr, (float*)s->st_w,
+                                         (float*)s->st_ya, s->mom_u, s->mom_part, dt / 6, dt / 2, s->stream);
+            (void)hipEventRecord(e1, s->stream);
+            if (le != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+                (void)hipEventDestroy(e0);
+                (void)hipEventDestroy(e1);
+                s->plan = keep;
+                return fail(s, NUOSC_EHIP, "autotune launch failed");
+            }
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (t > 0) tmin = std::min(tmin, ms);
+        }
+        if (tmin < bms) {
+            bms = tmin;
+            best = zl;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    s->plan = plan_stage_zlen(s->geo, best);
+    drop_graph(s);
+    {
+        std::lock_guard<std::mutex> g(g_tune_mu);
+        g_tune_cache[key] = best;
+    }
+    if (best_zlen) *best_zlen = best;
+    if (best_ms) *best_ms = bms;
+    return NUOSC_OK;
 }
 
 int nuosc_calrhs(nuosc_sim* s, const double* in, double* out) {

@@ -62,6 +62,8 @@ struct StagePlan {
 int rows_per_wave(int esize);
 
 StagePlan plan_stage(const Geometry& g, int num_cus);
+// the same decomposition with a given z-segment length
+StagePlan plan_stage_zlen(const Geometry& g, int zlen);
 // The stage kernel addresses each field through a buffer descriptor: one
 // field must span fewer than 4 GiB.
 bool buffer_addressable(const Geometry& g, int esize);

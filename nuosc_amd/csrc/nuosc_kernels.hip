@@ -567,6 +567,18 @@ StagePlan plan_stage(const Geometry& g, int num_cus) {
     return p;
 }
 
+StagePlan plan_stage_zlen(const Geometry& g, int zlen) {
+    StagePlan p{};
+    const int R = kWave / g.lpr;
+    const long long quads = (g.nx + R - 1) / R;
+    p.zlen = std::max(1, std::min(zlen, g.nz));
+    p.nzs = (g.nz + p.zlen - 1) / p.zlen;
+    p.nquads = (int)quads;
+    p.ntask = (int)(quads * g.nch * p.nzs);
+    p.nblocks = (p.ntask + 3) / 4;
+    return p;
+}
+
 bool buffer_addressable(const Geometry& g, int esize) {
     return (long long)g.fs * esize < (1ll << 32) - 1;
 }

@@ -74,3 +74,21 @@ def test_estimate_matches_allocation(nuosc_mod):
             est = N.estimate_device_bytes(precision=p, **c)
             held = g.device_bytes()
             assert est >= held * 0.95 and est <= held * 1.5 + (1 << 20), (est, held)
+
+
+def test_autotune_keeps_state_and_results(nuosc_mod):
+    """The z-segment length changes only the decomposition: autotuning leaves
+    the state untouched and every zlen steps to the same bits."""
+    N = nuosc_mod
+    cfg = RUNS[1]
+    ref = make(N, cfg, 5, 64)
+    ref.step_rk4(6)
+    want = ref.get_state()
+    for zl in (8, 16, 3, 1000):
+        g = make(N, cfg, 5, 64)
+        before = g.get_state()
+        g.autotune(trials=1)
+        assert np.array_equal(g.get_state(), before)
+        g.set_zlen(zl)
+        g.step_rk4(6)
+        assert np.array_equal(g.get_state(), want), zl
