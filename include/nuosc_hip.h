@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define NUOSC_ABI_VERSION 1
+#define NUOSC_ABI_VERSION 2
 
 /* status codes */
 #define NUOSC_OK 0
@@ -48,7 +48,18 @@ extern "C" {
 #define NUOSC_VGRID_VORONOI 0 /* icosahedron.h weights (reference)          */
 #define NUOSC_VGRID_EQUAL 1   /* same directions, weights 4*pi/N            */
 
-#define NUOSC_BC_PERIODIC 0
+#define NUOSC_BC_PERIODIC 0 /* nuosc_boundary.cpp:4-52                                  */
+#define NUOSC_BC_OPEN_Z 1   /* open z ends, x periodic.  The reference's open boundary,
+                             * updateInjetOpenBoundary (nuosc_boundary.cpp:54-57), is a
+                             * stub; this one has no reference counterpart.  Before every
+                             * RK stage the gz ghost rows at each z end are refilled for
+                             * the stage time t: a direction entering the box there
+                             * (vz > 0 at z0, vz < 0 at z1) gets the ipt-0 state of
+                             * amplitude a = eps0 exp(-(z - vz t - zc)^2 / (2 sigma^2))
+                             * (eps0, sigma of the last fill_init if it used ipt 0, else
+                             * a = 0: the unperturbed state); a leaving direction gets the
+                             * quadratic extrapolation of the three edge cells.  Needs
+                             * nz >= 3 and gz in {2, 3}; not batchable. */
 
 typedef struct nuosc_sim nuosc_sim;
 
@@ -68,9 +79,11 @@ typedef struct nuosc_config {
     int renorm;      /* renormalise |P| after each step */
     int vgrid;       /* NUOSC_VGRID_* */
     int precision;   /* NUOSC_PREC_FP64 or NUOSC_PREC_FP32 */
-    int bc;          /* NUOSC_BC_PERIODIC */
+    int bc;          /* NUOSC_BC_PERIODIC or NUOSC_BC_OPEN_Z */
     int device;      /* HIP device ordinal */
     int use_graph;   /* capture one step as a hipGraph and replay it */
+    double zc;       /* centre of the ipt-0 profile eps0 exp(-(z-zc)^2/(2 sigma^2));
+                      * the reference's is 0 (nuosc_init.cpp:4, 98) */
 } nuosc_config;
 
 /* Diagnostics of one analysis() call (nuosc_ana.cpp:34-137).  The first nine
@@ -117,6 +130,7 @@ int nuosc_set_renorm(nuosc_sim* sim, int renorm);
 int nuosc_get_dims(const nuosc_sim* sim, int* nx, int* nz, int* nv, long long* lpts);
 int nuosc_get_scalars(const nuosc_sim* sim, double* dt, double* ds_L, double* phy_time,
                       double* n_nue0_2);
+/* Sets phy_time (also the time the open z boundary data is evaluated at). */
 int nuosc_set_phy_time(nuosc_sim* sim, double phy_time);
 /* CartGrid vx, vy, vz, vw (length nv) and X (nx), Z (nz) (CartGrid.h:29-32) */
 int nuosc_get_vgrid(const nuosc_sim* sim, double* vx, double* vy, double* vz, double* vw);
@@ -140,8 +154,9 @@ int nuosc_fill_init(nuosc_sim* sim, int ipt, double alpha, double eps0, double s
 int nuosc_step_rk4(nuosc_sim* sim, int nsteps);
 int nuosc_synchronize(nuosc_sim* sim);
 
-/* NuOsc::calRHS(out, in) (nuosc_class.cpp:11-307) with the periodic ghost
- * fill of its input (nuosc_boundary.cpp:4-52): in/out are host arrays of
+/* NuOsc::calRHS(out, in) (nuosc_class.cpp:11-307) with the boundary fill of
+ * its input (periodic: nuosc_boundary.cpp:4-52; open z: the NUOSC_BC_OPEN_Z
+ * rule at the current phy_time): in/out are host arrays of
  * 8*nx*nz*nv doubles in the interior layout.  Test primitive; uses the
  * handle's scratch state buffers (the state itself is left untouched). */
 int nuosc_calrhs(nuosc_sim* sim, const double* in, double* out);
@@ -156,7 +171,8 @@ int nuosc_renormalize(nuosc_sim* sim);
  * download_field: one field, interior layout (nx*nz*nv doubles).
  * download_field_ghosted: one field in the reference's ghosted layout with
  *   the ghost band filled by updatePeriodicBoundary's rule
- *   (nuosc_boundary.cpp:4-52) and zero corners ((nx+2gx)(nz+2gz)nv doubles).
+ *   (nuosc_boundary.cpp:4-52), or for NUOSC_BC_OPEN_Z by the open rule at the
+ *   current phy_time in z, and zero corners ((nx+2gx)(nz+2gz)nv doubles).
  * upload_field: one field from the interior layout.  After uploading all the
  *   fields call nuosc_state_updated to refresh the velocity moments. */
 int nuosc_download_field(nuosc_sim* sim, int field, double* host);
@@ -208,6 +224,14 @@ int nuosc_batch_size(const nuosc_batch* batch);
  * returns after the device work completed. */
 int nuosc_batch_step(nuosc_batch* batch, int nsteps);
 const char* nuosc_batch_last_error(const nuosc_batch* batch);
+/* z-segment length of the batched stage kernels, shared by every member
+ * (default 64, capped at each member's nz; 0 = each member's own plan).  A
+ * common length gives every block of a batched launch the same work.
+ * batch_autotune times the batched stage-1 kernel for zlen in {16, ..., 256}
+ * (trials launches each, the fastest kept; only the RK4 scratch copies are
+ * written) and keeps the best.  Results do not depend on zlen. */
+int nuosc_batch_set_zlen(nuosc_batch* batch, int zlen);
+int nuosc_batch_autotune(nuosc_batch* batch, int trials, int* best_zlen, double* best_ms);
 
 /* Host-only: device bytes a handle created from cfg would hold (admission
  * control for the serving loop). */

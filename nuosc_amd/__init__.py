@@ -27,6 +27,7 @@ DBL_MAX = np.finfo(np.float64).max
 NUOSC_OK, NUOSC_EINVAL, NUOSC_EHIP, NUOSC_ENOMEM, NUOSC_EUNSTABLE, NUOSC_ENODEV, NUOSC_EAGAIN = range(7)
 PREC_FP64, PREC_FP32 = 64, 32
 VGRID_VORONOI, VGRID_EQUAL = 0, 1
+BC_PERIODIC, BC_OPEN_Z = 0, 1
 
 
 class NuOscError(RuntimeError):
@@ -42,7 +43,7 @@ class Config(C.Structure):
         ("dx", C.c_double), ("dz", C.c_double),
         ("cfl", C.c_double), ("ko", C.c_double), ("mu", C.c_double), ("pmo", C.c_double),
         ("renorm", C.c_int), ("vgrid", C.c_int), ("precision", C.c_int), ("bc", C.c_int),
-        ("device", C.c_int), ("use_graph", C.c_int),
+        ("device", C.c_int), ("use_graph", C.c_int), ("zc", C.c_double),
     ]
 
 
@@ -130,6 +131,8 @@ SIGNATURES = {
     "nuosc_batch_size": ([_vp], C.c_int),
     "nuosc_batch_step": ([_vp, C.c_int], C.c_int),
     "nuosc_batch_last_error": ([_vp], C.c_char_p),
+    "nuosc_batch_set_zlen": ([_vp, C.c_int], C.c_int),
+    "nuosc_batch_autotune": ([_vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)], C.c_int),
     "nuosc_estimate_device_bytes": ([C.POINTER(Config), C.POINTER(C.c_longlong)], C.c_int),
     "nuosc_server_create": ([C.c_int, C.c_int, C.c_longlong, C.c_int, C.POINTER(_vp)], C.c_int),
     "nuosc_server_submit": ([_vp, C.POINTER(Request), C.POINTER(C.c_longlong)], C.c_int),
@@ -175,11 +178,14 @@ class NuOsc:
 
     def __init__(self, nv=4, gx=2, gz=2, x0=-1.0, x1=1.0, z0=-10.0, z1=10.0, dx=0.1, dz=0.1,
                  CFL=0.4, ko=0.0, mu=1.0, pmo=0.1, renorm=False, precision=PREC_FP64,
-                 vgrid_mode=VGRID_VORONOI, device=0, use_graph=True):
+                 vgrid_mode=VGRID_VORONOI, device=0, use_graph=True, bc=0, zc=0.0):
+        """bc: BC_PERIODIC (the reference's boundary) or BC_OPEN_Z (open z
+        ends with the ipt-0 inflow pulse, x periodic); zc: centre of the ipt-0
+        profile (0 in the reference)."""
         L = lib()
         cfg = Config(nn=nv, gx=gx, gz=gz, x0=x0, x1=x1, z0=z0, z1=z1, dx=dx, dz=dz, cfl=CFL, ko=ko,
-                     mu=mu, pmo=pmo, renorm=int(renorm), vgrid=vgrid_mode, precision=precision, bc=0,
-                     device=device, use_graph=int(use_graph))
+                     mu=mu, pmo=pmo, renorm=int(renorm), vgrid=vgrid_mode, precision=precision, bc=bc,
+                     device=device, use_graph=int(use_graph), zc=zc)
         h = _vp()
         rc = L.nuosc_create(C.byref(cfg), C.byref(h))
         if rc != 0:
@@ -245,6 +251,9 @@ class NuOsc:
     @property
     def phy_time(self):
         return self._scalars()[2]
+
+    def set_phy_time(self, t):
+        self._check(lib().nuosc_set_phy_time(self._h, float(t)))
 
     @property
     def n_nue0(self):
@@ -386,6 +395,16 @@ class Batch:
 
     def step(self, nsteps=1):
         self._check(lib().nuosc_batch_step(self._h, int(nsteps)))
+
+    def set_zlen(self, zlen):
+        """z-segment length shared by all members (0: each member's own plan)."""
+        self._check(lib().nuosc_batch_set_zlen(self._h, int(zlen)))
+
+    def autotune(self, trials=2):
+        """Pick the fastest shared z-segment length; returns (zlen, stage-1 ms)."""
+        z, ms = C.c_int(), C.c_double()
+        self._check(lib().nuosc_batch_autotune(self._h, int(trials), C.byref(z), C.byref(ms)))
+        return z.value, ms.value
 
     def close(self):
         if getattr(self, "_h", None):

@@ -26,7 +26,8 @@ void NuOsc::check(int rc, const char* what) {
 }
 
 NuOsc::NuOsc(int px, int pz, int nv, int nphi, int gx, int gz, double x0, double x1, double z0, double z1,
-             double dx_, double dz_, double CFL_, double ko_, int precision, int vgrid, bool use_graph, int device)
+             double dx_, double dz_, double CFL_, double ko_, int precision, int vgrid, bool use_graph, int device,
+             int bc, double zc)
     : dx(dx_), dz(dz_), CFL(CFL_), ko(ko_) {
     (void)nphi;  // polar velocity grid only
     if (px != 1 || pz != 1)
@@ -48,7 +49,8 @@ NuOsc::NuOsc(int px, int pz, int nv, int nphi, int gx, int gz, double x0, double
     c.renorm = 0;
     c.vgrid = vgrid;
     c.precision = precision;
-    c.bc = NUOSC_BC_PERIODIC;
+    c.bc = bc;
+    c.zc = zc;
     c.device = device;
     c.use_graph = use_graph ? 1 : 0;
     int rc = nuosc_create(&c, &sim_);
@@ -88,7 +90,10 @@ NuOsc::NuOsc(int px, int pz, int nv, int nphi, int gx, int gz, double x0, double
     std::printf("            z:( %12f %12f )  dz = %g\n", z0, z1, dz);
     std::printf("   Fieldvar: 8 -- device memory %.2f GB\n", bytes / 1024. / 1024. / 1024.);
     std::printf("   dt = %g     CFL = %g\n", dt, CFL);
-    std::printf("   Boundary: Periodic\n");
+    if (bc == NUOSC_BC_OPEN_Z)
+        std::printf("   Boundary: x periodic, z open (ipt-0 inflow pulse centred at z = %g)\n", zc);
+    else
+        std::printf("   Boundary: Periodic\n");
     std::printf("   Velocity grid: Icosahedron sphere grid%s\n", vgrid == NUOSC_VGRID_EQUAL ? " (equal weights)" : "");
     std::printf("   Dissipation: 3-th order KO w/ eps = %g\n", ko);
     std::printf("   Advection: ON (Center-FD)\n");
@@ -99,10 +104,13 @@ NuOsc::NuOsc(int px, int pz, int nv, int nphi, int gx, int gz, double x0, double
     if (!anafile_) std::cout << "*** Open fails: " << "./analysis.dat" << std::endl;
     anafile_ << "### [ phy_time,   1:maxrelP,    2:surv, survb,    4:avgP, avgPb,      6:aM0    7:Lem   8:ELNe]"
              << std::endl;
+    growthfile_.open("growth.dat", std::ofstream::out | std::ofstream::trunc);
+    growthfile_ << "### [ phy_time,   1:|rho_ex| = sqrt(ds_L * sum w (emr^2 + emi^2)) ]" << std::endl;
 }
 
 NuOsc::~NuOsc() {
     anafile_.close();
+    growthfile_.close();
     nuosc_destroy(sim_);
 }
 
@@ -165,6 +173,7 @@ nuosc_analysis_t NuOsc::analysis() {
     anafile_ << a.phy_time << std::setprecision(13) << " " << a.maxdP << " " << a.surv << " " << a.survb << " "
              << a.avgP << " " << a.avgPb << " " << a.aM0 << " " << a.Lex << " " << a.ELNe << " " << std::endl;
     first_ana_line_ = false;
+    growthfile_ << std::setprecision(13) << a.phy_time << " " << std::sqrt(a.rho_ex2) << std::endl;
     if (a.status != NUOSC_OK) {
         // the reference asserts maxdP < 10 (nuosc_ana.cpp:131); a NaN state also stops here
         std::fprintf(stderr, "analysis: maxdP = %g (non-finite points: %g) -- the run is unstable, stopping.\n",

@@ -10,6 +10,10 @@
 // Changed: px, pz (MPI decomposition) must be 1; snapshots name their
 // quantity ("P3" or a field name) instead of holding a raw host pointer,
 // and read the device state at the time of the dump.
+// Added: precision, velocity-grid weights, boundary (periodic or open z
+// with an inflowing ipt-0 pulse centred at zc) and, beside analysis.dat
+// (whose columns stay the reference's), growth.dat with |rho_ex| per
+// analysis call for linear growth rates.
 #pragma once
 
 #include <cmath>
@@ -56,7 +60,7 @@ public:
 
     NuOsc(int px, int pz, int nv, int nphi, int gx, int gz, double x0, double x1, double z0, double z1, double dx,
           double dz, double CFL, double ko, int precision = NUOSC_PREC_FP64, int vgrid = NUOSC_VGRID_VORONOI,
-          bool use_graph = true, int device = 0);
+          bool use_graph = true, int device = 0, int bc = NUOSC_BC_PERIODIC, double zc = 0.0);
     ~NuOsc();
     NuOsc(const NuOsc&) = delete;
     NuOsc& operator=(const NuOsc&) = delete;
@@ -90,7 +94,7 @@ public:
 
 private:
     nuosc_sim* sim_ = nullptr;
-    std::ofstream anafile_;
+    std::ofstream anafile_, growthfile_;
     std::list<NuOscSnapShot> snapshots_;
     bool first_ana_line_ = true;
     std::vector<double> G0_, G0b_;

@@ -51,6 +51,8 @@ struct nuosc_sim {
     double* d_part = nullptr;
     double* d_ana = nullptr;
     double* d_tmp = nullptr;    // FP64 staging (interior or ghosted field)
+    double* d_time = nullptr;   // device copy of phy_time (open boundary data)
+    Inflow inflow{};
     size_t tmp_bytes = 0;
     long long device_bytes = 0;
     hipStream_t stream = nullptr;
@@ -122,18 +124,30 @@ int enqueue_step(nuosc_sim* s, hipEvent_t* ev = nullptr) {
     const Geometry& g = s->geo;
     const bool comp = sizeof(T) == 4;
     double* P = s->mom_part;
+    // open z boundary: ghost rows of each stage input at the stage time
+    // t, t + dt/2, t + dt/2, t + dt (t read from the device copy)
+    const bool open = g.bc == NUOSC_BC_OPEN_Z;
+    auto ghosts = [&](T* y, double cdt) {
+        return open ? launch_open_ghosts<T>(g, y, s->d_g0, s->d_g0 + s->nv, s->inflow, s->d_time, cdt, s->stream)
+                    : hipSuccess;
+    };
     if (ev) HIPCK(s, hipEventRecord(ev[0], s->stream));
+    HIPCK(s, ghosts(u, 0.0));
     HIPCK(s, launch_stage<T>(1, g, s->ph, s->plan, u, nullptr, w, ya, s->mom_u, P, c6, c2, s->stream));
     HIPCK(s, launch_moment_totals(g, comp, P, s->mom_a, s->stream));
     if (ev) HIPCK(s, hipEventRecord(ev[1], s->stream));
+    HIPCK(s, ghosts(ya, c2));
     HIPCK(s, launch_stage<T>(2, g, s->ph, s->plan, ya, u, w, yb, s->mom_a, P, c3, c2, s->stream));
     HIPCK(s, launch_moment_totals(g, comp, P, s->mom_b, s->stream));
     if (ev) HIPCK(s, hipEventRecord(ev[2], s->stream));
+    HIPCK(s, ghosts(yb, c2));
     HIPCK(s, launch_stage<T>(3, g, s->ph, s->plan, yb, u, w, ya, s->mom_b, P, c3, dt, s->stream));
     HIPCK(s, launch_moment_totals(g, comp, P, s->mom_a, s->stream));
     if (ev) HIPCK(s, hipEventRecord(ev[3], s->stream));
+    HIPCK(s, ghosts(ya, dt));
     HIPCK(s, launch_stage<T>(4, g, s->ph, s->plan, ya, nullptr, w, u, s->mom_a, P, c6, 0.0, s->stream));
     HIPCK(s, launch_moment_totals(g, comp, P, s->mom_u, s->stream));
+    if (open) HIPCK(s, launch_advance_time(s->d_time, dt, s->stream));
     if (ev) HIPCK(s, hipEventRecord(ev[4], s->stream));
     if (s->cfg.renorm) {
         HIPCK(s, launch_renorm<T>(g, u, s->d_g0, s->d_g0 + s->nv, s->stream));
@@ -165,6 +179,25 @@ int analysis_sums(nuosc_sim* s, double* h) {
     return NUOSC_OK;
 }
 
+// open z boundary: ghost rows of the state u at the current time
+int fill_state_ghosts(nuosc_sim* s, void* y) {
+    if (s->geo.bc != NUOSC_BC_OPEN_Z) return NUOSC_OK;
+    if (is64(s))
+        HIPCK(s, launch_open_ghosts<double>(s->geo, (double*)y, s->d_g0, s->d_g0 + s->nv, s->inflow, s->d_time, 0.0,
+                                            s->stream));
+    else
+        HIPCK(s, launch_open_ghosts<float>(s->geo, (float*)y, s->d_g0, s->d_g0 + s->nv, s->inflow, s->d_time, 0.0,
+                                           s->stream));
+    return NUOSC_OK;
+}
+
+int set_device_time(nuosc_sim* s) {
+    if (!s->d_time) return NUOSC_OK;
+    HIPCK(s, hipMemcpyAsync(s->d_time, &s->phy_time, sizeof(double), hipMemcpyHostToDevice, s->stream));
+    HIPCK(s, hipStreamSynchronize(s->stream));
+    return NUOSC_OK;
+}
+
 int check_field(nuosc_sim* s, int f) {
     if (f < 0 || f >= kNVar) return fail(s, NUOSC_EINVAL, "field index out of range");
     return NUOSC_OK;
@@ -177,7 +210,8 @@ extern "C" {
 int nuosc_abi_version(void) { return NUOSC_ABI_VERSION; }
 
 const char* nuosc_build_info(void) {
-    return "libnuosc_hip: gfx950 RK4 stage kernels (fp64, fp32 + fp64 moments), abi " "1";
+    return "libnuosc_hip: gfx950 RK4 stage kernels (fp64, fp32 + fp64 moments; periodic and open-z boundaries), "
+           "abi 2";
 }
 
 int nuosc_vgrid(int nn, int mode, double* vx, double* vy, double* vz, double* vw) {
@@ -204,7 +238,10 @@ int nuosc_create(const nuosc_config* cfg, nuosc_sim** out) {
         return fail(nullptr, NUOSC_EINVAL, "bad box or spacing");
     if (c.precision != NUOSC_PREC_FP64 && c.precision != NUOSC_PREC_FP32)
         return fail(nullptr, NUOSC_EINVAL, "precision must be 64 or 32");
-    if (c.bc != NUOSC_BC_PERIODIC) return fail(nullptr, NUOSC_EINVAL, "only periodic boundaries are supported");
+    if (c.bc != NUOSC_BC_PERIODIC && c.bc != NUOSC_BC_OPEN_Z)
+        return fail(nullptr, NUOSC_EINVAL, "bc must be NUOSC_BC_PERIODIC or NUOSC_BC_OPEN_Z");
+    if (c.bc == NUOSC_BC_OPEN_Z && c.gz > 3)
+        return fail(nullptr, NUOSC_EINVAL, "the open z boundary takes gz = 2 or 3");
     if (c.vgrid != NUOSC_VGRID_VORONOI && c.vgrid != NUOSC_VGRID_EQUAL)
         return fail(nullptr, NUOSC_EINVAL, "unknown velocity grid");
 
@@ -221,6 +258,12 @@ int nuosc_create(const nuosc_config* cfg, nuosc_sim** out) {
         delete s;
         return fail(nullptr, NUOSC_EINVAL, "empty grid");
     }
+    if (c.bc == NUOSC_BC_OPEN_Z && s->nz < 3) {
+        delete s;
+        return fail(nullptr, NUOSC_EINVAL, "the open z boundary needs nz >= 3 (quadratic outflow extrapolation)");
+    }
+    const int zoff = (c.bc == NUOSC_BC_OPEN_Z) ? c.gz : 0;
+    const int nzs = s->nz + 2 * zoff;
     s->X.resize(s->nx);
     s->Z.resize(s->nz);
     for (int i = 0; i < s->nx; ++i) s->X[i] = c.x0 + (i + 0.5) * c.dx;
@@ -232,7 +275,7 @@ int nuosc_create(const nuosc_config* cfg, nuosc_sim** out) {
     s->nvp = (s->nv + align - 1) / align * align;
     const int lpr = kWave / rows_per_wave((int)s->esize);
     s->nch = (s->nv + lpr - 1) / lpr;
-    s->fs = (long long)s->nx * s->nz * s->nvp;
+    s->fs = (long long)s->nx * nzs * s->nvp;
     if (s->fs * (long long)s->esize >= (1ll << 32) - 1) {
         delete s;
         return fail(nullptr, NUOSC_EINVAL, "one field must stay below 4 GiB (nx*nz*nv too large for one GPU handle)");
@@ -292,6 +335,13 @@ int nuosc_create(const nuosc_config* cfg, nuosc_sim** out) {
     s->geo.vz = s->d_vgrid + 2 * s->nv;
     s->geo.vw = s->d_vgrid + 3 * s->nv;
     s->geo.bc = c.bc;
+    s->geo.nzs = nzs;
+    s->geo.zoff = zoff;
+    s->inflow = Inflow{0, 0.0, 1.0, c.zc, c.z0, c.dz};
+    if (c.bc == NUOSC_BC_OPEN_Z) {
+        if (hipMalloc((void**)&s->d_time, sizeof(double)) != hipSuccess) return bail(NUOSC_ENOMEM, "allocation failed");
+        if (hipMemsetAsync(s->d_time, 0, sizeof(double), s->stream) != hipSuccess) return bail(NUOSC_EHIP, "memset failed");
+    }
     const int nb = ana_blocks(s->geo);
     if (hipMalloc((void**)&s->d_part, (size_t)nb * kNAna * 2 * sizeof(double)) != hipSuccess ||
         hipMalloc((void**)&s->d_ana, kNAna * sizeof(double)) != hipSuccess)
@@ -320,7 +370,7 @@ int nuosc_destroy(nuosc_sim* s) {
     if (s->stream) hipStreamSynchronize(s->stream);
     drop_graph(s);
     void* bufs[] = {s->st_u, s->st_w, s->st_ya, s->st_yb, s->mom_part, s->mom_u, s->mom_a, s->mom_b,
-                    s->d_vgrid, s->d_g0, s->d_part, s->d_ana, s->d_tmp};
+                    s->d_vgrid, s->d_g0, s->d_part, s->d_ana, s->d_tmp, s->d_time};
     for (void* p : bufs)
         if (p) hipFree(p);
     if (s->ev0) hipEventDestroy(s->ev0);
@@ -377,7 +427,7 @@ int nuosc_get_scalars(const nuosc_sim* s, double* dt, double* ds_L, double* phy_
 int nuosc_set_phy_time(nuosc_sim* s, double t) {
     if (!s) return NUOSC_EINVAL;
     s->phy_time = t;
-    return NUOSC_OK;
+    return set_device_time(s);
 }
 
 int nuosc_get_vgrid(const nuosc_sim* s, double* vx, double* vy, double* vz, double* vw) {
@@ -421,7 +471,7 @@ int nuosc_fill_init(nuosc_sim* s, int ipt, double alpha, double eps0, double sig
     for (int j = 0; j < s->nz; ++j) {
         const double z = s->Z[j];
         double tmpr = 0.0;
-        if (ipt == 0) tmpr = eps0 * std::exp(-(z - 0.0) * (z - 0.0) / (2.0 * sigma * sigma));
+        if (ipt == 0) tmpr = eps0 * std::exp(-(z - s->cfg.zc) * (z - s->cfg.zc) / (2.0 * sigma * sigma));
         else if (ipt == 2) tmpr = eps0 * (1.0 + std::cos(2 * M_PI * (z - 0.0) / (2.0 * sigma * sigma))) * 0.5;
         else if (ipt == 3) tmpr = eps0;
         tab[j] = tmpr;
@@ -447,6 +497,13 @@ int nuosc_fill_init(nuosc_sim* s, int ipt, double alpha, double eps0, double sig
     s->n_nue0[0] = h[ANA_SURV] * s->ds_L;
     s->n_nue0[1] = h[ANA_SURVB] * s->ds_L;
     s->phy_time = 0.0;
+    // open z boundary: the inflow pulse continues the ipt-0 profile
+    s->inflow.on = (ipt == 0) ? 1 : 0;
+    s->inflow.eps0 = eps0;
+    s->inflow.sigma = sigma;
+    drop_graph(s);
+    rc = set_device_time(s);
+    if (rc) return rc;
     rc = refresh_moments(s);
     if (rc) return rc;
     HIPCK(s, hipStreamSynchronize(s->stream));
@@ -633,6 +690,8 @@ int nuosc_calrhs(nuosc_sim* s, const double* in, double* out) {
         if (is64(s)) HIPCK(s, launch_unpack<double>(s->geo, s->d_tmp, field_ptr<double>(s->st_ya, s, f), s->stream));
         else HIPCK(s, launch_unpack<float>(s->geo, s->d_tmp, field_ptr<float>(s->st_ya, s, f), s->stream));
     }
+    rc = fill_state_ghosts(s, s->st_ya);
+    if (rc) return rc;
     if (is64(s)) {
         HIPCK(s, launch_moments<double>(s->geo, (double*)s->st_ya, s->mom_part, s->stream));
         HIPCK(s, launch_moment_totals(s->geo, false, s->mom_part, s->mom_a, s->stream));
@@ -708,6 +767,8 @@ int nuosc_download_field_ghosted(nuosc_sim* s, int f, double* host) {
         sizeof(double) * (size_t)(s->nx + 2 * s->cfg.gx) * (s->nz + 2 * s->cfg.gz) * s->nv;
     rc = ensure_tmp(s, bytes);
     if (rc) return rc;
+    rc = fill_state_ghosts(s, s->st_u);
+    if (rc) return rc;
     if (is64(s))
         HIPCK(s, launch_ghosted<double>(s->geo, s->cfg.gx, s->cfg.gz, field_ptr<double>(s->st_u, s, f), s->d_tmp,
                                         s->stream));
@@ -780,6 +841,10 @@ struct nuosc_batch {
     bool dirty = true;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
+    // z-segment length shared by all members (0: each member's own plan).
+    // One length gives every block of the batched launch the same amount of
+    // work, so the contiguous block ranges of the 8 XCDs stay balanced.
+    int zlen = 64;
     std::string err;
 };
 
@@ -797,6 +862,42 @@ void batch_drop_graph(nuosc_batch* b) {
 }  // namespace
 
 extern "C" {
+
+namespace {
+// (re)build the device tables of the batched launches from the members
+int batch_tables(nuosc_batch* b) {
+    std::vector<BatchMember> ms;
+    for (nuosc_sim* m : b->members) {
+        BatchMember bm;
+        bm.g = m->geo;
+        bm.ph = m->ph;
+        bm.pl = b->zlen > 0 ? plan_stage_zlen(m->geo, std::min(b->zlen, m->nz)) : m->plan;
+        bm.u = m->st_u;
+        bm.w = m->st_w;
+        bm.ya = m->st_ya;
+        bm.yb = m->st_yb;
+        bm.mpart = m->mom_part;
+        bm.mu = m->mom_u;
+        bm.ma = m->mom_a;
+        bm.mb = m->mom_b;
+        bm.dt = m->dt;
+        ms.push_back(bm);
+    }
+    hipError_t e = batch_build(b->tables, ms, is64(b->members[0]) ? 8 : 4, b->stream);
+    if (e != hipSuccess) return bfail(b, NUOSC_EHIP, std::string("batch tables: ") + hipGetErrorString(e));
+    return NUOSC_OK;
+}
+
+// members' own queued work (init, uploads, moments) completes first
+int batch_sync_members(nuosc_batch* b) {
+    for (nuosc_sim* m : b->members) {
+        if (m->cfg.renorm) return bfail(b, NUOSC_EINVAL, "renormalising runs cannot be batched");
+        if (!m->have_moments && refresh_moments(m) != NUOSC_OK) return bfail(b, NUOSC_EHIP, m->err);
+        if (hipStreamSynchronize(m->stream) != hipSuccess) return bfail(b, NUOSC_EHIP, "member stream sync failed");
+    }
+    return NUOSC_OK;
+}
+}  // namespace
 
 int nuosc_batch_create(int device, int precision, nuosc_batch** out) {
     if (!out) return NUOSC_EINVAL;
@@ -835,6 +936,7 @@ int nuosc_batch_add(nuosc_batch* b, nuosc_sim* s) {
     if (s->cfg.device != b->device || s->cfg.precision != b->precision)
         return bfail(b, NUOSC_EINVAL, "member must use the batch's device and precision");
     if (s->cfg.renorm) return bfail(b, NUOSC_EINVAL, "renormalising runs cannot be batched");
+    if (s->cfg.bc != NUOSC_BC_PERIODIC) return bfail(b, NUOSC_EINVAL, "only periodic runs can be batched");
     for (nuosc_sim* m : b->members)
         if (m == s) return bfail(b, NUOSC_EINVAL, "already a member");
     b->members.push_back(s);
@@ -859,33 +961,12 @@ int nuosc_batch_size(const nuosc_batch* b) { return b ? (int)b->members.size() :
 int nuosc_batch_step(nuosc_batch* b, int nsteps) {
     if (!b || nsteps < 0) return NUOSC_EINVAL;
     if (b->members.empty() || nsteps == 0) return NUOSC_OK;
-    // members' own queued work (init, uploads, moments) completes first
-    for (nuosc_sim* m : b->members) {
-        if (m->cfg.renorm) return bfail(b, NUOSC_EINVAL, "renormalising runs cannot be batched");
-        if (!m->have_moments && refresh_moments(m) != NUOSC_OK) return bfail(b, NUOSC_EHIP, m->err);
-        if (hipStreamSynchronize(m->stream) != hipSuccess) return bfail(b, NUOSC_EHIP, "member stream sync failed");
-    }
+    int rc0 = batch_sync_members(b);
+    if (rc0) return rc0;
     if (b->dirty) {
         batch_drop_graph(b);
-        std::vector<BatchMember> ms;
-        for (nuosc_sim* m : b->members) {
-            BatchMember bm;
-            bm.g = m->geo;
-            bm.ph = m->ph;
-            bm.pl = m->plan;
-            bm.u = m->st_u;
-            bm.w = m->st_w;
-            bm.ya = m->st_ya;
-            bm.yb = m->st_yb;
-            bm.mpart = m->mom_part;
-            bm.mu = m->mom_u;
-            bm.ma = m->mom_a;
-            bm.mb = m->mom_b;
-            bm.dt = m->dt;
-            ms.push_back(bm);
-        }
-        hipError_t e = batch_build(b->tables, ms, is64(b->members[0]) ? 8 : 4, b->stream);
-        if (e != hipSuccess) return bfail(b, NUOSC_EHIP, std::string("batch tables: ") + hipGetErrorString(e));
+        int rc = batch_tables(b);
+        if (rc) return rc;
         if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
             return bfail(b, NUOSC_EHIP, "capture failed");
         hipError_t le = batch_step(b->tables, b->stream);
@@ -908,6 +989,59 @@ int nuosc_batch_step(nuosc_batch* b, int nsteps) {
     return NUOSC_OK;
 }
 
+int nuosc_batch_set_zlen(nuosc_batch* b, int zlen) {
+    if (!b || zlen < 0) return NUOSC_EINVAL;
+    (void)hipStreamSynchronize(b->stream);
+    b->zlen = zlen;
+    b->dirty = true;
+    return NUOSC_OK;
+}
+
+int nuosc_batch_autotune(nuosc_batch* b, int trials, int* best_zlen, double* best_ms) {
+    if (!b || trials < 1) return NUOSC_EINVAL;
+    if (b->members.empty()) return bfail(b, NUOSC_EINVAL, "empty batch");
+    int rc = batch_sync_members(b);
+    if (rc) return rc;
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+        return bfail(b, NUOSC_EHIP, "event creation failed");
+    const int keep = b->zlen;
+    int best = keep;
+    float bms = 1e30f;
+    for (int zl = 16; zl <= 256; zl *= 2) {
+        b->zlen = zl;
+        rc = batch_tables(b);
+        if (rc) break;
+        float tmin = 1e30f;
+        for (int t = 0; t < trials + 1 && rc == NUOSC_OK; ++t) {  // first launch warms up
+            (void)hipEventRecord(e0, b->stream);
+            // stage 1 writes only the RK4 scratch copies (w, yA), never the state
+            hipError_t le = batch_stage(b->tables, 1, b->stream);
+            (void)hipEventRecord(e1, b->stream);
+            if (le != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+                rc = bfail(b, NUOSC_EHIP, "autotune launch failed");
+                break;
+            }
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (t > 0) tmin = std::min(tmin, ms);
+        }
+        if (rc == NUOSC_OK && tmin < bms) {
+            bms = tmin;
+            best = zl;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    b->zlen = rc == NUOSC_OK ? best : keep;
+    b->dirty = true;
+    batch_drop_graph(b);
+    if (rc) return rc;
+    if (best_zlen) *best_zlen = best;
+    if (best_ms) *best_ms = bms;
+    return NUOSC_OK;
+}
+
 int nuosc_estimate_device_bytes(const nuosc_config* c, long long* bytes) {
     if (!c || !bytes || c->nn < 1 || c->nn > 64 || !(c->dx > 0) || !(c->dz > 0)) return NUOSC_EINVAL;
     const long long nx = (long long)((c->x1 - c->x0) / c->dx), nz = (long long)((c->z1 - c->z0) / c->dz);
@@ -916,7 +1050,8 @@ int nuosc_estimate_device_bytes(const nuosc_config* c, long long* bytes) {
     const int esize = c->precision == NUOSC_PREC_FP32 ? 4 : 8;
     const long long align = 128 / esize, nvp = (nv + align - 1) / align * align;
     const long long lpr = kWave / rows_per_wave(esize), nch = (nv + lpr - 1) / lpr;
-    long long b = 4ll * kNVar * nx * nz * nvp * esize;          // u, w, yA, yB
+    const long long nzs = nz + (c->bc == NUOSC_BC_OPEN_Z ? 2 * c->gz : 0);
+    long long b = 4ll * kNVar * nx * nzs * nvp * esize;         // u, w, yA, yB
     b += nx * nz * nch * kMomRec * 8 + 3ll * nx * nz * kMomStride * 8;  // moments
     b += 4 * nv * 8 + 2 * nv * 8 + 2048ll * kNAna * 16 + kNAna * 8;    // v-grid, G0, analysis
     b += 2 * nz * 8;                                                   // init tables

@@ -3,9 +3,12 @@
 //
 // Device data layout (one state copy):
 //   8 fields (ee, mm, emr, emi, bee, bmm, bemr, bemi) one after another, each
-//   field [nx][nz][nvp] with v fastest; nvp = nv rounded up so that every site
-//   row starts on a 128-byte boundary.  There is no ghost band: the periodic
-//   boundary is an index wrap inside the stencil loads.
+//   field [nx][nzs][nvp] with v fastest; nvp = nv rounded up so that every
+//   site row starts on a 128-byte boundary.  Periodic runs store no ghost
+//   band (nzs = nz): the boundary is an index wrap inside the stencil loads.
+//   Runs with an open z boundary store zoff = gz ghost rows at each z end
+//   (nzs = nz + 2 gz, interior row j at stored row j + zoff), refilled before
+//   every stage; x stays periodic by index wrap.
 //   Velocity moments: 12 doubles {A0..A3, B0..B3, C0..C3} per site with
 //   A = sum w u (bemr-emr), B = sum w u (bemi+emi), C = sum w u
 //   (bee-bmm-ee+mm), u = (1, vx, vy, vz).  The stage kernels write partial
@@ -39,7 +42,15 @@ struct Geometry {
     double twelve_dx, twelve_dz;   // 12*dx, 12*dz (advection denominators)
     double ko_x, ko_z;     // -ko/dx/16, -ko/dz/16
     const double *vx, *vy, *vz, *vw;   // device, length nv
-    int bc;                // 0 periodic
+    int bc;                // 0 periodic, 1 open in z
+    int nzs, zoff;         // stored z rows per x row, stored row of interior z = 0
+};
+
+// Inflow data of the open z boundary: the ipt-0 pulse
+// a = eps0 exp(-(z - vz t - zc)^2 / (2 sigma^2)) (on = 0: a = 0).
+struct Inflow {
+    int on;
+    double eps0, sigma, zc, z0, dz;
 };
 
 struct Physics {
@@ -54,7 +65,7 @@ struct Physics {
 // mtot holds the moment totals of yin; mpart (stages 1-4) receives the partial
 // moments of yout.
 struct StagePlan {
-    int zlen, nzs, nquads, ntask, nblocks;
+    int zlen, nseg, nquads, ntask, nblocks;   // z-segment length and count, x-row quads, waves, blocks
 };
 
 // x-rows per wavefront of the stage kernel (FP64: 4 rows x 16 directions,
@@ -134,6 +145,17 @@ struct BatchTables {
 hipError_t batch_build(BatchTables& t, const std::vector<BatchMember>& members, int esize, hipStream_t s);
 void batch_free(BatchTables& t);
 hipError_t batch_step(const BatchTables& t, hipStream_t s);
+// one stage kernel of every member (no moment totals): autotuner timing
+hipError_t batch_stage(const BatchTables& t, int stage, hipStream_t s);
+
+// Open z boundary: fill the zoff ghost rows at both z ends of state y for
+// time t = *t_dev + cdt (inflow directions: the Inflow pulse state;
+// outflow directions: quadratic extrapolation of the three edge cells).
+template <typename T>
+hipError_t launch_open_ghosts(const Geometry& g, T* y, const double* G0, const double* G0b, const Inflow& in,
+                              const double* t_dev, double cdt, hipStream_t s);
+// *t_dev += dt (device copy of the physical time, read by the boundary fill)
+hipError_t launch_advance_time(double* t_dev, double dt, hipStream_t s);
 
 // Renormalisation of the state to |P| = 1 (the reference's formula).
 template <typename T>

@@ -41,6 +41,12 @@ __device__ __forceinline__ int wrapi(int k, int n) {
     return k < 0 ? k + n : k;
 }
 
+// stored site row of interior site (i*nz + j): i*nzs + j + zoff
+__device__ __forceinline__ long long sto(const Geometry& g, long long site) {
+    if (g.zoff == 0) return site;
+    return (site / g.nz) * g.nzs + site % g.nz + g.zoff;
+}
+
 // Blocks are dealt to the 8 XCDs round-robin (block b runs on XCD b % 8).
 // Return the logical block index so that each XCD works on one contiguous
 // range of logical blocks (a bijection for any grid size).
@@ -64,79 +70,6 @@ __device__ __forceinline__ void two_sum(double a, double b, double& s, double& e
     s = a + b;
     const double bb = s - a;
     e = (a - (s - bb)) + (b - bb);
-}
-
-// Wave-wide sums of 16 values with 17 cross-lane exchanges: after the call
-// lane 4*m + r (r = 0..3) holds the total of a[m] over the 64 lanes.
-template <bool COMP>
-__device__ __forceinline__ double wave_sum16(const double (&a)[16], int lane) {
-    if constexpr (!COMP) {
-        double b[8], c[4], d[2];
-        const bool h5 = lane & 32, h4 = lane & 16, h3 = lane & 8, h2 = lane & 4;
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const double send = h5 ? a[k] : a[k + 8];
-            const double keep = h5 ? a[k + 8] : a[k];
-            b[k] = keep + __shfl_xor(send, 32);
-        }
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const double send = h4 ? b[k] : b[k + 4];
-            const double keep = h4 ? b[k + 4] : b[k];
-            c[k] = keep + __shfl_xor(send, 16);
-        }
-#pragma unroll
-        for (int k = 0; k < 2; ++k) {
-            const double send = h3 ? c[k] : c[k + 2];
-            const double keep = h3 ? c[k + 2] : c[k];
-            d[k] = keep + __shfl_xor(send, 8);
-        }
-        double e = (h2 ? d[1] : d[0]) + __shfl_xor(h2 ? d[0] : d[1], 4);
-        e += __shfl_xor(e, 2);
-        e += __shfl_xor(e, 1);
-        return e;
-    } else {
-        // same exchange pattern carrying (sum, error) pairs
-        double bs[8], be[8], cs[4], ce[4], ds[2], de[2];
-        const bool h5 = lane & 32, h4 = lane & 16, h3 = lane & 8, h2 = lane & 4;
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const double send = h5 ? a[k] : a[k + 8];
-            const double keep = h5 ? a[k + 8] : a[k];
-            two_sum(keep, __shfl_xor(send, 32), bs[k], be[k]);
-        }
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const double ss = h4 ? bs[k] : bs[k + 4], se = h4 ? be[k] : be[k + 4];
-            const double ks = h4 ? bs[k + 4] : bs[k], ke = h4 ? be[k + 4] : be[k];
-            double err;
-            two_sum(ks, __shfl_xor(ss, 16), cs[k], err);
-            ce[k] = ke + __shfl_xor(se, 16) + err;
-        }
-#pragma unroll
-        for (int k = 0; k < 2; ++k) {
-            const double ss = h3 ? cs[k] : cs[k + 2], se = h3 ? ce[k] : ce[k + 2];
-            const double ks = h3 ? cs[k + 2] : cs[k], ke = h3 ? ce[k + 2] : ce[k];
-            double err;
-            two_sum(ks, __shfl_xor(ss, 8), ds[k], err);
-            de[k] = ke + __shfl_xor(se, 8) + err;
-        }
-        double es, ee, err;
-        {
-            const double ss = h2 ? ds[0] : ds[1], se = h2 ? de[0] : de[1];
-            const double ks = h2 ? ds[1] : ds[0], ke = h2 ? de[1] : de[0];
-            two_sum(ks, __shfl_xor(ss, 4), es, err);
-            ee = ke + __shfl_xor(se, 4) + err;
-        }
-#pragma unroll
-        for (int m = 2; m >= 1; m >>= 1) {
-            double s2;
-            two_sum(es, __shfl_xor(es, m), s2, err);
-            ee = ee + __shfl_xor(ee, m) + err;
-            es = s2;
-        }
-        return es + ee;
-    }
 }
 
 // Per-lane moment terms of one point: q = (w, vx w, vy w, vz w) times the
@@ -245,7 +178,7 @@ struct StageArgs {
     double* mpart;            // partial moments of yout: [site][nch][kMomRec]
     double cw, cy;
     double mu2, mu, p2st, p2ct, pmo, ct2, st;
-    int zlen, ntask, nquads;
+    int zlen, nquads;
 };
 
 // Sums of 16 values over each group of L consecutive lanes (L = 16, 32, 64):
@@ -315,7 +248,9 @@ struct Slab {
     T v[2][kNVar][NS * L];   // two z levels (by parity) of all 8 fields
 };
 
-// One wavefront's work: task = 4 * lblock + wave-in-block.
+// One wavefront's work.  Block lblock = (direction chunk, group of 4
+// x-row quads, z segment), chunk fastest; wave wib of the block takes quad
+// 4 * group + wib.
 template <typename T, int R, int FPD, int STAGE, bool COMP>
 __device__ __forceinline__ void stage_body(const StageArgs<T, FPD>& a, int lblock, int wib, Slab<T, R>& slab) {
     constexpr int L = kWave / R;   // directions per row
@@ -323,19 +258,20 @@ __device__ __forceinline__ void stage_body(const StageArgs<T, FPD>& a, int lbloc
     auto& xs = slab.v;
 
     const int lane = threadIdx.x & 63;
-    const int task = lblock * 4 + wib;
-    if (task >= a.ntask) return;
-
     const int nx = a.g.nx, nz = a.g.nz, nch = a.g.nch, nv = a.g.nv;
+    const int nzs = a.g.nzs, zoff = a.g.zoff;             // stored rows per x row, ghost rows below z = 0
     const unsigned rowb = (unsigned)a.g.nvp * sizeof(T);  // bytes per site row
-    // direction chunk fastest: the waves of one block (and of neighbouring
-    // blocks, which the XCD remap keeps together) read adjacent 128-byte
-    // segments of the same site rows, and the x-row quads next to each other
-    // run at the same time (their halo reads hit in L2)
-    const int ch = task % nch;
-    const int rest = task / nch;
-    const int qd = rest % a.nquads;
-    const int zs = rest / a.nquads;
+    // The four waves of a block sweep 4R adjacent x rows of one chunk side by
+    // side, so most halo rows a wave reads are rows its neighbours in the
+    // block read at about the same time (L2 hits).  Chunk fastest across
+    // blocks: neighbouring blocks, which the XCD remap keeps on one XCD, read
+    // adjacent 128-byte segments of the same site rows.
+    const int ch = lblock % nch;
+    const int rest = lblock / nch;
+    const int nqg = (a.nquads + 3) >> 2;
+    const int qd = (rest % nqg) * 4 + wib;
+    const int zs = rest / nqg;
+    if (qd >= a.nquads) return;   // last group of a grid with nquads % 4 != 0
     const int i0 = qd * R;
     const int g = lane / L, dl = lane % L;
     const int v = ch * L + dl;
@@ -350,8 +286,12 @@ __device__ __forceinline__ void stage_body(const StageArgs<T, FPD>& a, int lbloc
     int rmin = wrapi(i0 - 2, nx);
     rmin = min(rmin, wrapi(i0 - 1, nx));
     for (int q = 0; q < R + 2; ++q) rmin = min(rmin, wrapi(i0 + q, nx));
-    const unsigned rbase = (unsigned)rmin * nz;
-    const unsigned fieldb = (unsigned)nz * rowb;  // bytes per x-row of one field
+    const unsigned rbase = (unsigned)rmin * nzs + zoff;   // stored row of (rmin, z = 0)
+    const unsigned mbase = (unsigned)rmin * nz;           // moment record of (rmin, z = 0)
+    const unsigned fieldb = (unsigned)nzs * rowb;  // bytes per x-row of one field
+    // stored-row offset of interior z index jj (periodic: wrapped; open: the
+    // ghost rows hold jj in [-zoff, nz + zoff))
+    auto zrow = [&](int jj) { return (unsigned)(wrapi(jj + zoff, nzs) - zoff); };
     const unsigned vo = (unsigned)(rw - rmin) * fieldb + (unsigned)vl * sizeof(T);
     unsigned vh[NH];
     int slot_h[NH];
@@ -383,17 +323,17 @@ __device__ __forceinline__ void stage_body(const StageArgs<T, FPD>& a, int lbloc
     T hreg[NH][kNVar]; // halo rows loaded for z = j+2, staged into LDS one iteration later
 #pragma unroll
     for (int qz = 0; qz < 4; ++qz) {
-        const unsigned so = (rbase + (unsigned)wrapi(j0 - 2 + qz, nz)) * rowb;
+        const unsigned so = (rbase + zrow(j0 - 2 + qz)) * rowb;
 #pragma unroll
         for (int f = 0; f < kNVar; ++f) win[f][qz] = a.yin.ld(a.yin.base, f, vo, so);
     }
     {
-        const unsigned so = (rbase + (unsigned)wrapi(j0, nz)) * rowb;
+        const unsigned so = (rbase + zrow(j0)) * rowb;
 #pragma unroll
         for (int h = 0; h < NH; ++h)
 #pragma unroll
             for (int f = 0; f < kNVar; ++f) xs[j0 & 1][f][slot_h[h]] = a.yin.ld(a.yin.base, f, vh[h], so);
-        const unsigned s1 = (rbase + (unsigned)wrapi(j0 + 1, nz)) * rowb;
+        const unsigned s1 = (rbase + zrow(j0 + 1)) * rowb;
 #pragma unroll
         for (int h = 0; h < NH; ++h)
 #pragma unroll
@@ -412,7 +352,7 @@ __device__ __forceinline__ void stage_body(const StageArgs<T, FPD>& a, int lbloc
         // ---- all loads of the iteration
         const unsigned long long by = a.yin.fresh_base();
         {
-            const unsigned s2 = (rbase + (unsigned)wrapi(j + 2, nz)) * rowb;
+            const unsigned s2 = (rbase + zrow(j + 2)) * rowb;
 #pragma unroll
             for (int f = 0; f < kNVar; ++f) win[f][4] = a.yin.ld(by, f, vo, s2);
 #pragma unroll
@@ -438,7 +378,7 @@ __device__ __forceinline__ void stage_body(const StageArgs<T, FPD>& a, int lbloc
             asm volatile("" : "+s"(bm));
             const __amdgpu_buffer_rsrc_t rm =
                 __builtin_amdgcn_make_buffer_rsrc((void*)bm, 0, (int)a.mtot_bytes, kRsrcWord3);
-            const unsigned sm = (rbase + j) * (kMomStride * 8);
+            const unsigned sm = (mbase + j) * (kMomStride * 8);
 #pragma unroll
             for (int s = 0; s < kNMom; s += 2) {
                 const u32x4 x = __builtin_amdgcn_raw_buffer_load_b128(rm, vom + s * 8, sm, 0);
@@ -560,10 +500,10 @@ StagePlan plan_stage(const Geometry& g, int num_cus) {
     while (zlen > 8 && rows * ((g.nz + zlen - 1) / zlen) < want) zlen = (zlen + 1) / 2;
     if (zlen > 64 && rows * ((g.nz + 63) / 64) >= want) zlen = 64;
     p.zlen = zlen;
-    p.nzs = (g.nz + zlen - 1) / zlen;
+    p.nseg = (g.nz + zlen - 1) / zlen;
     p.nquads = (int)quads;
-    p.ntask = (int)(rows * p.nzs);
-    p.nblocks = (p.ntask + 3) / 4;
+    p.nblocks = (int)(((quads + 3) / 4) * g.nch * p.nseg);
+    p.ntask = p.nblocks * 4;
     return p;
 }
 
@@ -572,10 +512,10 @@ StagePlan plan_stage_zlen(const Geometry& g, int zlen) {
     const int R = kWave / g.lpr;
     const long long quads = (g.nx + R - 1) / R;
     p.zlen = std::max(1, std::min(zlen, g.nz));
-    p.nzs = (g.nz + p.zlen - 1) / p.zlen;
+    p.nseg = (g.nz + p.zlen - 1) / p.zlen;
     p.nquads = (int)quads;
-    p.ntask = (int)(quads * g.nch * p.nzs);
-    p.nblocks = (p.ntask + 3) / 4;
+    p.nblocks = (int)(((quads + 3) / 4) * g.nch * p.nseg);
+    p.ntask = p.nblocks * 4;
     return p;
 }
 
@@ -624,7 +564,6 @@ static StageArgs<T, FPD> make_stage_args(const Geometry& g, const Physics& ph, c
     a.ct2 = 2 * ph.ct;
     a.st = ph.st;
     a.zlen = pl.zlen;
-    a.ntask = pl.ntask;
     a.nquads = pl.nquads;
     return a;
 }
@@ -835,6 +774,34 @@ static hipError_t batch_step_t(const BatchTables& t, hipStream_t s) {
     return hipGetLastError();
 }
 
+template <typename T, int FPD>
+static hipError_t batch_stage_t(const BatchTables& t, int st, hipStream_t s) {
+    constexpr int R = sizeof(T) == 8 ? 4 : 2;
+    constexpr bool COMP = sizeof(T) == 4;
+    const dim3 grid(t.nblocks), block(256);
+    const StageArgs<T, FPD>* a = (const StageArgs<T, FPD>*)t.args[st - 1];
+    switch (st) {
+        case 1: hipLaunchKernelGGL((k_stage_batch<T, R, FPD, 1, COMP>), grid, block, 0, s, a, t.bmap); break;
+        case 2: hipLaunchKernelGGL((k_stage_batch<T, R, FPD, 2, COMP>), grid, block, 0, s, a, t.bmap); break;
+        case 3: hipLaunchKernelGGL((k_stage_batch<T, R, FPD, 3, COMP>), grid, block, 0, s, a, t.bmap); break;
+        default: hipLaunchKernelGGL((k_stage_batch<T, R, FPD, 4, COMP>), grid, block, 0, s, a, t.bmap); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t batch_stage(const BatchTables& t, int stage, hipStream_t s) {
+    if (t.nblocks == 0) return hipSuccess;
+    if (stage < 1 || stage > 4) return hipErrorInvalidValue;
+    if (t.esize == 8) {
+        if (t.fpd == 4) return batch_stage_t<double, 4>(t, stage, s);
+        if (t.fpd == 2) return batch_stage_t<double, 2>(t, stage, s);
+        return batch_stage_t<double, 1>(t, stage, s);
+    }
+    if (t.fpd == 4) return batch_stage_t<float, 4>(t, stage, s);
+    if (t.fpd == 2) return batch_stage_t<float, 2>(t, stage, s);
+    return batch_stage_t<float, 1>(t, stage, s);
+}
+
 hipError_t batch_step(const BatchTables& t, hipStream_t s) {
     if (t.nblocks == 0) return hipSuccess;
     if (t.esize == 8) {
@@ -863,10 +830,11 @@ __global__ __launch_bounds__(256) void k_moments(const Geometry g, const T* __re
 #pragma unroll
     for (int s = 0; s < kNMom; ++s) { S[s] = 0.0; E[s] = 0.0; }
     const int v1 = min(g.nv, (ch + 1) * g.lpr);
+    const long long so = sto(g, site) * g.nvp;
     for (int v = ch * g.lpr; v < v1; ++v) {
         T yv[kNVar];
 #pragma unroll
-        for (int f = 0; f < kNVar; ++f) yv[f] = y[f * g.fs + site * g.nvp + v];
+        for (int f = 0; f < kNVar; ++f) yv[f] = y[f * g.fs + so + v];
         const double vw = g.vw[v];
         const double q[4] = {vw, g.vx[v] * vw, g.vy[v] * vw, g.vz[v] * vw};
         double t[16];
@@ -917,9 +885,10 @@ __global__ __launch_bounds__(256) void k_init(const Geometry g, int ipt, const d
     if (n >= total) return;
     const long long site = n / g.nvp;
     const int v = (int)(n % g.nvp);
+    const long long o = sto(g, site) * g.nvp + v;
     if (v >= g.nv) {  // row padding
 #pragma unroll
-        for (int f = 0; f < kNVar; ++f) y[f * g.fs + n] = (T)0;
+        for (int f = 0; f < kNVar; ++f) y[f * g.fs + o] = (T)0;
         return;
     }
     const int j = (int)(site % g.nz);
@@ -939,14 +908,14 @@ __global__ __launch_bounds__(256) void k_init(const Geometry g, int ipt, const d
         p3o = p3o_z[j];
     }
     const double hg = 0.5 * G0[v], hgb = 0.5 * G0b[v];
-    y[0 * g.fs + n] = (T)(hg * (1.0 + p3o));
-    y[1 * g.fs + n] = (T)(hg * (1.0 - p3o));
-    y[2 * g.fs + n] = (T)(hg * tmpr);
-    y[3 * g.fs + n] = (T)(ipt == 5 ? hg * tmpi : 0.0);
-    y[4 * g.fs + n] = (T)(hgb * (1.0 + p3o));
-    y[5 * g.fs + n] = (T)(hgb * (1.0 - p3o));
-    y[6 * g.fs + n] = (T)(hgb * tmpr);
-    y[7 * g.fs + n] = (T)(ipt == 5 ? -hgb * tmpi : 0.0);
+    y[0 * g.fs + o] = (T)(hg * (1.0 + p3o));
+    y[1 * g.fs + o] = (T)(hg * (1.0 - p3o));
+    y[2 * g.fs + o] = (T)(hg * tmpr);
+    y[3 * g.fs + o] = (T)(ipt == 5 ? hg * tmpi : 0.0);
+    y[4 * g.fs + o] = (T)(hgb * (1.0 + p3o));
+    y[5 * g.fs + o] = (T)(hgb * (1.0 - p3o));
+    y[6 * g.fs + o] = (T)(hgb * tmpr);
+    y[7 * g.fs + o] = (T)(ipt == 5 ? -hgb * tmpi : 0.0);
 }
 
 template <typename T>
@@ -987,7 +956,7 @@ __global__ __launch_bounds__(kAnaThreads) void k_analysis(const Geometry g, cons
     for (long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x; n < pts; n += stride) {
         const long long site = n / g.nv;
         const int v = (int)(n % g.nv);
-        const long long o = site * g.nvp + v;
+        const long long o = sto(g, site) * g.nvp + v;
         double f[kNVar];
         bool finite = true;
 #pragma unroll
@@ -1103,8 +1072,9 @@ __global__ void k_ghosted(const Geometry g, int gx, int gz, const T* __restrict_
     const int i = ig - gx, j = jg - gz;
     const bool xin = (i >= 0 && i < g.nx), zin = (j >= 0 && j < g.nz);
     if (!xin && !zin) { dst[n] = 0.0; return; }  // corners are never filled
-    const int iw = wrapi(i, g.nx), jw = wrapi(j, g.nz);
-    dst[n] = (double)fld[((long long)iw * g.nz + jw) * g.nvp + v];
+    // x periodic by wrap; z periodic by wrap, or (open z) the stored ghost rows
+    const int iw = wrapi(i, g.nx), jw = g.zoff ? j : wrapi(j, g.nz);
+    dst[n] = (double)fld[((long long)iw * g.nzs + jw + g.zoff) * g.nvp + v];
 }
 
 template <typename T>
@@ -1122,7 +1092,7 @@ __global__ void k_pack(const Geometry g, const T* __restrict__ fld, double* __re
     if (n >= total) return;
     const long long s = n / g.nv;
     const int v = (int)(n % g.nv);
-    dst[n] = (double)fld[s * g.nvp + v];
+    dst[n] = (double)fld[sto(g, s) * g.nvp + v];
 }
 
 template <typename T>
@@ -1132,7 +1102,7 @@ __global__ void k_unpack(const Geometry g, const double* __restrict__ src, T* __
     if (n >= total) return;
     const long long s = n / g.nvp;
     const int v = (int)(n % g.nvp);
-    fld[n] = (v < g.nv) ? (T)src[s * g.nv + v] : (T)0;
+    fld[sto(g, s) * g.nvp + v] = (v < g.nv) ? (T)src[s * g.nv + v] : (T)0;
 }
 
 template <typename T>
@@ -1160,7 +1130,7 @@ __global__ void k_p3_slice(const Geometry g, const T* __restrict__ y, const doub
     const long long r = n / sv;
     const int j = (int)(r % g.nz), a = (int)(r / g.nz);
     const int v = vi[b];
-    const long long o = ((long long)xi[a] * g.nz + j) * g.nvp + v;
+    const long long o = ((long long)xi[a] * g.nzs + j + g.zoff) * g.nvp + v;
     const double iG = 1.0 / G0[v];  // nuosc_ana.cpp:10,14
     dst[n] = ((double)y[0 * g.fs + o] - (double)y[1 * g.fs + o]) * iG;
 }
@@ -1184,29 +1154,107 @@ __global__ void k_renorm(const Geometry g, T* __restrict__ y, const double* __re
     if (n >= total) return;
     const int v = (int)(n % g.nvp);
     if (v >= g.nv) return;
+    const long long o = sto(g, n / g.nvp) * g.nvp + v;
     double f[kNVar];
 #pragma unroll
-    for (int q = 0; q < kNVar; ++q) f[q] = (double)y[q * g.fs + n];
+    for (int q = 0; q < kNVar; ++q) f[q] = (double)y[q * g.fs + o];
     const double iG = 1.0 / G0[v], iGb = 1.0 / G0b[v];
     const double P1 = f[2] * iG, P2 = -f[3] * iG, P3 = (f[0] - f[1]) * iG;
     const double P1b = f[6] * iGb, P2b = f[7] * iGb, P3b = (f[4] - f[5]) * iGb;
     const double iP = 1.0 / sqrt(P1 * P1 + P2 * P2 + P3 * P3);
     const double iPb = 1.0 / sqrt(P1b * P1b + P2b * P2b + P3b * P3b);
     const double tmp = iP * P3 * G0[v], tmpb = iPb * P3b * G0b[v];
-    y[0 * g.fs + n] = (T)(G0[v] + tmp);
-    y[1 * g.fs + n] = (T)(G0[v] - tmp);
-    y[2 * g.fs + n] = (T)(f[2] * iP);
-    y[3 * g.fs + n] = (T)(f[3] * iP);
-    y[4 * g.fs + n] = (T)(G0b[v] + tmpb);
-    y[5 * g.fs + n] = (T)(G0b[v] - tmpb);
-    y[6 * g.fs + n] = (T)(f[6] * iPb);
-    y[7 * g.fs + n] = (T)(f[7] * iPb);
+    y[0 * g.fs + o] = (T)(G0[v] + tmp);
+    y[1 * g.fs + o] = (T)(G0[v] - tmp);
+    y[2 * g.fs + o] = (T)(f[2] * iP);
+    y[3 * g.fs + o] = (T)(f[3] * iP);
+    y[4 * g.fs + o] = (T)(G0b[v] + tmpb);
+    y[5 * g.fs + o] = (T)(G0b[v] - tmpb);
+    y[6 * g.fs + o] = (T)(f[6] * iPb);
+    y[7 * g.fs + o] = (T)(f[7] * iPb);
 }
 
 template <typename T>
 hipError_t launch_renorm(const Geometry& g, T* y, const double* G0, const double* G0b, hipStream_t s) {
     const long long total = (long long)g.nx * g.nz * g.nvp;
     hipLaunchKernelGGL((k_renorm<T>), dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, g, y, G0, G0b);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// open z boundary (no reference counterpart: updateInjetOpenBoundary,
+// nuosc_boundary.cpp:54-57, is a stub)
+// ---------------------------------------------------------------------------
+// One thread per (x row, side, ghost depth, direction) writes all 8 fields.
+template <typename T>
+__global__ __launch_bounds__(256) void k_open_ghosts(const Geometry g, T* __restrict__ y,
+                                                     const double* __restrict__ G0,
+                                                     const double* __restrict__ G0b, const Inflow in,
+                                                     const double* __restrict__ t_dev, double cdt) {
+#pragma clang fp contract(off)
+    const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long total = (long long)g.nx * 2 * g.zoff * g.nv;
+    if (n >= total) return;
+    const int v = (int)(n % g.nv);
+    long long r = n / g.nv;
+    const int q = (int)(r % g.zoff) + 1;  // ghost depth 1..zoff
+    r /= g.zoff;
+    const int side = (int)(r % 2);
+    const int i = (int)(r / 2);
+    const int j = side == 0 ? -q : g.nz - 1 + q;
+    const long long o = ((long long)i * g.nzs + j + g.zoff) * g.nvp + v;
+    const double vz = g.vz[v];
+    const bool inflow = side == 0 ? (vz > 0.0) : (vz < 0.0);
+    if (inflow) {
+        double a = 0.0;
+        if (in.on) {
+            const double t = t_dev[0] + cdt;
+            const double z = in.z0 + (j + 0.5) * in.dz;
+            const double zz = z - vz * t - in.zc;
+            a = in.eps0 * exp(-zz * zz / (2.0 * in.sigma * in.sigma));
+        }
+        const double p3o = sqrt(1.0 - a * a);
+        const double hg = 0.5 * G0[v], hgb = 0.5 * G0b[v];
+        y[0 * g.fs + o] = (T)(hg * (1.0 + p3o));
+        y[1 * g.fs + o] = (T)(hg * (1.0 - p3o));
+        y[2 * g.fs + o] = (T)(hg * a);
+        y[3 * g.fs + o] = (T)0.0;
+        y[4 * g.fs + o] = (T)(hgb * (1.0 + p3o));
+        y[5 * g.fs + o] = (T)(hgb * (1.0 - p3o));
+        y[6 * g.fs + o] = (T)(hgb * a);
+        y[7 * g.fs + o] = (T)0.0;
+    } else {
+        // quadratic extrapolation through the three edge cells
+        const double w0 = q == 1 ? 3.0 : (q == 2 ? 6.0 : 10.0);
+        const double w1 = q == 1 ? -3.0 : (q == 2 ? -8.0 : -15.0);
+        const double w2 = q == 1 ? 1.0 : (q == 2 ? 3.0 : 6.0);
+        const int e0 = side == 0 ? 0 : g.nz - 1, dir = side == 0 ? 1 : -1;
+        const long long c0 = ((long long)i * g.nzs + e0 + g.zoff) * g.nvp + v;
+        const long long step = (long long)dir * g.nvp;
+#pragma unroll
+        for (int f = 0; f < kNVar; ++f) {
+            const T* p = y + f * g.fs + c0;
+            y[f * g.fs + o] = (T)(w0 * (double)p[0] + w1 * (double)p[step] + w2 * (double)p[2 * step]);
+        }
+    }
+}
+
+template <typename T>
+hipError_t launch_open_ghosts(const Geometry& g, T* y, const double* G0, const double* G0b, const Inflow& in,
+                              const double* t_dev, double cdt, hipStream_t s) {
+    if (g.zoff < 2 || g.zoff > 3 || g.nz < 3) return hipErrorInvalidValue;
+    const long long total = (long long)g.nx * 2 * g.zoff * g.nv;
+    hipLaunchKernelGGL((k_open_ghosts<T>), dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, g, y, G0,
+                       G0b, in, t_dev, cdt);
+    return hipGetLastError();
+}
+
+__global__ void k_advance_time(double* t_dev, double dt) {
+    if (threadIdx.x == 0) t_dev[0] = t_dev[0] + dt;
+}
+
+hipError_t launch_advance_time(double* t_dev, double dt, hipStream_t s) {
+    hipLaunchKernelGGL(k_advance_time, dim3(1), dim3(64), 0, s, t_dev, dt);
     return hipGetLastError();
 }
 
@@ -1225,7 +1273,9 @@ hipError_t launch_renorm(const Geometry& g, T* y, const double* G0, const double
     template hipError_t launch_unpack<T>(const Geometry&, const double*, T*, hipStream_t);               \
     template hipError_t launch_p3_slice<T>(const Geometry&, const T*, const double*, const int*, int,    \
                                            const int*, int, double*, hipStream_t);                       \
-    template hipError_t launch_renorm<T>(const Geometry&, T*, const double*, const double*, hipStream_t);
+    template hipError_t launch_renorm<T>(const Geometry&, T*, const double*, const double*, hipStream_t); \
+    template hipError_t launch_open_ghosts<T>(const Geometry&, T*, const double*, const double*, const Inflow&, \
+                                              const double*, double, hipStream_t);
 NUOSC_INST(double)
 NUOSC_INST(float)
 #undef NUOSC_INST

@@ -2,7 +2,9 @@
 // NuOsc class.  Same flags, defaults and outputs (stdout analysis lines,
 // analysis.dat, wall-time lines, [Summ] line).  Additions: --precision 64|32,
 // --seed S (for --ipt 5), --vw voronoi|equal, --snap_p3 1 (P3 .bin/.meta
-// snapshot every DUMP_EVERY steps), --graph 0|1, --device N.
+// snapshot every DUMP_EVERY steps), --graph 0|1, --device N, --bc
+// periodic|open (open: z ends open, an ipt-0 pulse flowing in) and --zc Z
+// (centre of the ipt-0 profile, 0 in the reference).
 #include <sys/resource.h>
 
 #include <chrono>
@@ -40,6 +42,8 @@ int main(int argc, char* argv[]) {
     int gz = 2, gx = 2;
     // additions
     int precision = NUOSC_PREC_FP64, vgrid = NUOSC_VGRID_VORONOI, snap_p3 = 0, use_graph = 1, device = 0;
+    int bc = NUOSC_BC_PERIODIC;
+    double zc = 0.0;
     unsigned long long seed = 12345;
 
     for (int t = 1; t < argc; t++) {
@@ -91,6 +95,13 @@ int main(int argc, char* argv[]) {
         else if (!std::strcmp(a, "--snap_p3")) snap_p3 = std::atoi(val());
         else if (!std::strcmp(a, "--graph")) use_graph = std::atoi(val());
         else if (!std::strcmp(a, "--device")) device = std::atoi(val());
+        else if (!std::strcmp(a, "--bc")) {
+            const char* m = val();
+            if (!std::strcmp(m, "periodic")) bc = NUOSC_BC_PERIODIC;
+            else if (!std::strcmp(m, "open")) bc = NUOSC_BC_OPEN_Z;
+            else { std::printf("Unreconganized parameters %s!\n", m); std::exit(0); }
+        }
+        else if (!std::strcmp(a, "--zc")) zc = std::atof(val());
         else {
             // nuosc.cpp:135-137 (message and exit status kept)
             std::printf("Unreconganized parameters %s!\n", a);
@@ -100,7 +111,7 @@ int main(int argc, char* argv[]) {
 
     try {
         NuOsc state(px, pz, nv_in, nphi, gx, gz, x0, x1, z0, z1, dx, dz, cfl, ko, precision, vgrid, use_graph != 0,
-                    device);
+                    device, bc, zc);
         const long long lpts = state.grid.get_lpts();
         state.set_mu(mu);
         state.set_pmo(pmo);

@@ -236,6 +236,14 @@ void orc_set_physics(orc_sim* s, double mu, double pmo, int renorm) {
     s->renorm = renorm;
 }
 
+int orc_set_boundary(orc_sim* s, int bc, double zc) {
+    if (bc != ORC_BC_PERIODIC && bc != ORC_BC_OPEN_Z) return -1;
+    if (bc == ORC_BC_OPEN_Z && (s->nz < 3 || s->gz > 3)) return -1;
+    s->bc = bc;
+    s->zc = zc;
+    return 0;
+}
+
 /* ------------------------------------------------------------------------- */
 /* Initial data (nuosc_init.cpp:4-137)                                        */
 /* ------------------------------------------------------------------------- */
@@ -291,8 +299,8 @@ int orc_fill_init(orc_sim* s, int ipt, double alpha, double eps0, double sigma, 
                 long ijv = orc_idx(s, i, j, v);
                 const double G0 = s->G0[v], G0b = s->G0b[v];
                 double tmpr, tmpi = 0.0, z = s->Z[j];
-                if (ipt == 0)      /* eps_c, nuosc_init.cpp:4 */
-                    tmpr = eps0 * exp(-(z - 0.0) * (z - 0.0) / (2.0 * sigma * sigma));
+                if (ipt == 0)      /* eps_c, nuosc_init.cpp:4 (centre zc, 0 there) */
+                    tmpr = eps0 * exp(-(z - s->zc) * (z - s->zc) / (2.0 * sigma * sigma));
                 else if (ipt == 2) /* eps_p, nuosc_init.cpp:6 */
                     tmpr = eps0 * (1.0 + cos(2 * M_PI * (z - 0.0) / (2.0 * sigma * sigma))) * 0.5;
                 else if (ipt == 3)
@@ -329,6 +337,9 @@ int orc_fill_init(orc_sim* s, int ipt, double alpha, double eps0, double sigma, 
     s->n_nue0[1] = n01 * s->ds_L;
     s->phy_time = 0.0;
     s->analysis_lines = 0;
+    s->inflow_on = (ipt == 0);
+    s->inflow_eps0 = eps0;
+    s->inflow_sigma = sigma;
     return 0;
 }
 
@@ -357,6 +368,70 @@ void orc_periodic_ghosts(orc_sim* s, double* const f[ORC_NVAR]) {
                 for (int q = 0; q < ORC_NVAR; ++q) f[q][l0] = f[q][l1];
                 for (int q = 0; q < ORC_NVAR; ++q) f[q][r0] = f[q][r1];
             }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Open z boundary (no reference counterpart: updateInjetOpenBoundary,        */
+/* nuosc_boundary.cpp:54-57, is a stub).                                      */
+/* ------------------------------------------------------------------------- */
+void orc_open_ghosts(orc_sim* s, double* const f[ORC_NVAR], double t) {
+    const int nx = s->nx, nz = s->nz, nv = s->nv, gz = s->gz;
+    /* quadratic extrapolation weights of interior cells 0,1,2 (counted inward)
+     * for ghost depth 1, 2, 3 */
+    static const double W[3][3] = {{3.0, -3.0, 1.0}, {6.0, -8.0, 3.0}, {10.0, -15.0, 6.0}};
+#pragma omp parallel for
+    for (int i = 0; i < nx; ++i)
+        for (int side = 0; side < 2; ++side)
+            for (int q = 1; q <= gz; ++q) {
+                const int j = side == 0 ? -q : nz - 1 + q;
+                const int e0 = side == 0 ? 0 : nz - 1, dir = side == 0 ? 1 : -1;
+                const double z = s->z0 + (j + 0.5) * s->dz;
+                for (int v = 0; v < nv; ++v) {
+                    const long g = orc_idx(s, i, j, v);
+                    const double vz = s->vz[v];
+                    const int inflow = side == 0 ? (vz > 0.0) : (vz < 0.0);
+                    if (inflow) {
+                        double a = 0.0;
+                        if (s->inflow_on) {
+                            const double zz = z - vz * t - s->zc;
+                            a = s->inflow_eps0 * exp(-zz * zz / (2.0 * s->inflow_sigma * s->inflow_sigma));
+                        }
+                        const double p3o = sqrt(1.0 - a * a);
+                        const double G0 = s->G0[v], G0b = s->G0b[v];
+                        f[0][g] = 0.5 * G0 * (1.0 + p3o);
+                        f[1][g] = 0.5 * G0 * (1.0 - p3o);
+                        f[2][g] = 0.5 * G0 * a;
+                        f[3][g] = 0.0;
+                        f[4][g] = 0.5 * G0b * (1.0 + p3o);
+                        f[5][g] = 0.5 * G0b * (1.0 - p3o);
+                        f[6][g] = 0.5 * G0b * a;
+                        f[7][g] = 0.0;
+                    } else {
+                        const long c0 = orc_idx(s, i, e0, v), c1 = orc_idx(s, i, e0 + dir, v),
+                                   c2 = orc_idx(s, i, e0 + 2 * dir, v);
+                        const double* w = W[q - 1];
+                        for (int k = 0; k < ORC_NVAR; ++k)
+                            f[k][g] = w[0] * f[k][c0] + w[1] * f[k][c1] + w[2] * f[k][c2];
+                    }
+                }
+            }
+    /* x ghosts periodic over the interior z rows (nuosc_boundary.cpp:29-50) */
+    for (int i = 0; i < s->gx; ++i)
+#pragma omp parallel for
+        for (int j = 0; j < nz; ++j)
+            for (int v = 0; v < nv; ++v) {
+                long l0 = orc_idx(s, -i - 1, j, v), l1 = orc_idx(s, nx - i - 1, j, v);
+                long r0 = orc_idx(s, nx + i, j, v), r1 = orc_idx(s, i, j, v);
+                for (int k = 0; k < ORC_NVAR; ++k) f[k][l0] = f[k][l1];
+                for (int k = 0; k < ORC_NVAR; ++k) f[k][r0] = f[k][r1];
+            }
+}
+
+void orc_fill_ghosts(orc_sim* s, double* const f[ORC_NVAR], double t) {
+    if (s->bc == ORC_BC_OPEN_Z)
+        orc_open_ghosts(s, f, t);
+    else
+        orc_periodic_ghosts(s, f);
 }
 
 /* ------------------------------------------------------------------------- */
@@ -460,25 +535,25 @@ static void swapbuf(double** a, double** b) {
 }
 
 void orc_step_rk4(orc_sim* s) {
-    const double dt = s->dt;
-    /* stage 1 */
-    orc_periodic_ghosts(s, s->stat);
+    const double dt = s->dt, t = s->phy_time;
+    /* stage 1 (boundary data at the stage times t, t+dt/2, t+dt/2, t+dt) */
+    orc_fill_ghosts(s, s->stat, t);
     orc_calrhs(s, s->rhs, s->stat);
     vec1(s, s->pre, s->stat, 0.5 * dt, s->rhs);
     /* stage 2 */
-    orc_periodic_ghosts(s, s->pre);
+    orc_fill_ghosts(s, s->pre, t + 0.5 * dt);
     orc_calrhs(s, s->cor, s->pre);
     vec1(s, s->rhs, s->rhs, 2.0, s->cor);
     vec1(s, s->cor, s->stat, 0.5 * dt, s->cor);
     swapbuf(s->pre, s->cor);
     /* stage 3 */
-    orc_periodic_ghosts(s, s->pre);
+    orc_fill_ghosts(s, s->pre, t + 0.5 * dt);
     orc_calrhs(s, s->cor, s->pre);
     vec1(s, s->rhs, s->rhs, 2.0, s->cor);
     vec1(s, s->cor, s->stat, dt, s->cor);
     swapbuf(s->pre, s->cor);
     /* stage 4 */
-    orc_periodic_ghosts(s, s->pre);
+    orc_fill_ghosts(s, s->pre, t + dt);
     orc_calrhs(s, s->cor, s->pre);
     vec2(s, s->pre, s->stat, 1.0 / 6.0 * dt, s->cor, s->rhs);
     swapbuf(s->pre, s->stat);

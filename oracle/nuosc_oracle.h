@@ -14,6 +14,11 @@
  *   orc_create (geometry)   CartGrid.cpp:191-208, nuosc_class.cpp:454-466
  *   orc_fill_init           nuosc_init.cpp:4-137 (ipt 0/2/3) + seeded ipt 5
  *   orc_periodic_ghosts     nuosc_boundary.cpp:4-52
+ *   orc_open_ghosts         open z boundary (the reference's
+ *                           updateInjetOpenBoundary, nuosc_boundary.cpp:54-57,
+ *                           is a stub): inflow ghosts carry the free-streaming
+ *                           ipt-0 pulse, outflow ghosts are quadratic
+ *                           extrapolations of the three edge cells
  *   orc_calrhs              nuosc_class.cpp:11-307 (2 flavours, KO_ORD_3,
  *                           central 4th-order advection, vacuum on)
  *   orc_step_rk4            nuosc_class.cpp:312-351, 374-427
@@ -34,6 +39,10 @@
  *   - init: sum_v w G0 = 1, sum_v w G0b = alpha, trace and purity identities.
  *   - analysis: an independent numpy formulation of the diagnostics plus the
  *     pure-state/conservation identities.
+ *   - open boundary (no reference counterpart): at mu=0 the exact solution is
+ *     free streaming of the initial profile; the interior error converges at
+ *     third order in dz (the order of the KO term) with inflow and outflow both
+ *     active; the z-sum of the RHS equals the telescoped boundary flux.
  * Nothing else of the reference builds here: every other reference source
  * includes common.h, which hard-wires COSENU_MPI and therefore <mpi.h>,
  * which this image does not have.
@@ -65,7 +74,15 @@ typedef struct orc_sim {
     /* four RK4 state buffers, 8 fields each (nuosc_class.h:106-156) */
     double *stat[ORC_NVAR], *rhs[ORC_NVAR], *pre[ORC_NVAR], *cor[ORC_NVAR];
     int analysis_lines;        /* analysis.dat line counter (first line precision quirk) */
+    /* boundary: 0 periodic (reference), 1 open in z (x periodic) */
+    int bc;
+    double zc;                 /* centre of the ipt-0 profile (the reference uses 0) */
+    int inflow_on;             /* inflow pulse active (fill_init with ipt 0) */
+    double inflow_eps0, inflow_sigma;
 } orc_sim;
+
+#define ORC_BC_PERIODIC 0
+#define ORC_BC_OPEN_Z 1
 
 /* Velocity grid.  Returns N = 10*nn*nn+2 and fills the four arrays (length N). */
 int orc_vgrid_icosahedron(int nn, double* vx, double* vy, double* vz, double* vw);
@@ -76,6 +93,9 @@ orc_sim* orc_create(int nn, int gx, int gz, double x0, double x1, double z0, dou
                     double dx, double dz, double cfl, double ko, int vgrid_mode);
 void orc_destroy(orc_sim* s);
 void orc_set_physics(orc_sim* s, double mu, double pmo, int renorm);
+/* Boundary mode and ipt-0 profile centre; call before orc_fill_init.  The open
+ * mode needs nz >= 3.  Returns 0, or -1 on bad input. */
+int orc_set_boundary(orc_sim* s, int bc, double zc);
 
 /* nuosc_init.cpp:17-137.  ipt 0,2,3 as the reference; ipt 5 = seeded complex
  * perturbation (counter-based RNG keyed by seed and the interior index).
@@ -89,6 +109,16 @@ void orc_seed_uniforms(uint64_t seed, long lin, double* ur, double* ui);
 long orc_idx(const orc_sim* s, int i, int j, int v);
 
 void orc_periodic_ghosts(orc_sim* s, double* const f[ORC_NVAR]);
+/* Open z boundary at time t, x periodic.  For ghost cell z (centre
+ * z0 + (j+1/2)dz, j < 0 or j >= nz) and direction v: if v enters the domain
+ * there (vz > 0 on the z0 side, vz < 0 on the z1 side) the ghost holds the
+ * ipt-0 state of amplitude a = eps0 exp(-(z - vz t - zc)^2 / (2 sigma^2))
+ * (a = 0, the unperturbed state, unless fill_init used ipt 0); otherwise it
+ * is the quadratic extrapolation of the three nearest interior cells
+ * (coefficients (3,-3,1), (6,-8,3), (10,-15,6) for ghost depth 1, 2, 3). */
+void orc_open_ghosts(orc_sim* s, double* const f[ORC_NVAR], double t);
+/* The boundary fill of the configured mode at time t. */
+void orc_fill_ghosts(orc_sim* s, double* const f[ORC_NVAR], double t);
 void orc_calrhs(const orc_sim* s, double* const out[ORC_NVAR], double* const in[ORC_NVAR]);
 void orc_step_rk4(orc_sim* s);
 void orc_renormalize(orc_sim* s, double* const f[ORC_NVAR]);

@@ -45,6 +45,8 @@ class _Sim(C.Structure):
         ("stat", C.POINTER(C.c_double) * 8), ("rhs", C.POINTER(C.c_double) * 8),
         ("pre", C.POINTER(C.c_double) * 8), ("cor", C.POINTER(C.c_double) * 8),
         ("analysis_lines", C.c_int),
+        ("bc", C.c_int), ("zc", C.c_double), ("inflow_on", C.c_int),
+        ("inflow_eps0", C.c_double), ("inflow_sigma", C.c_double),
     ]
 
 
@@ -65,6 +67,10 @@ def lib():
         L.orc_fill_init.restype = C.c_int
         L.orc_seed_uniforms.argtypes = [C.c_uint64, C.c_long, dp, dp]
         L.orc_periodic_ghosts.argtypes = [C.POINTER(_Sim), dp * 8]
+        L.orc_set_boundary.argtypes = [C.POINTER(_Sim), C.c_int, C.c_double]
+        L.orc_set_boundary.restype = C.c_int
+        L.orc_open_ghosts.argtypes = [C.POINTER(_Sim), dp * 8, C.c_double]
+        L.orc_fill_ghosts.argtypes = [C.POINTER(_Sim), dp * 8, C.c_double]
         L.orc_calrhs.argtypes = [C.POINTER(_Sim), dp * 8, dp * 8]
         L.orc_step_rk4.argtypes = [C.POINTER(_Sim)]
         L.orc_renormalize.argtypes = [C.POINTER(_Sim), dp * 8]
@@ -115,15 +121,19 @@ class OracleSim:
     """The oracle's NuOsc: reference layout and operation order, FP64."""
 
     BUF_STAT, BUF_RHS, BUF_PRE, BUF_COR = 0, 1, 2, 3
+    BC_PERIODIC, BC_OPEN_Z = 0, 1
 
     def __init__(self, nn=4, gx=2, gz=2, x0=-3.2, x1=3.2, z0=-3.2, z1=3.2, dx=0.1, dz=0.1,
-                 cfl=0.4, ko=0.1, mu=1.0, pmo=0.0, renorm=0, vgrid_mode=0):
+                 cfl=0.4, ko=0.1, mu=1.0, pmo=0.0, renorm=0, vgrid_mode=0, bc=0, zc=0.0):
         L = lib()
         p = L.orc_create(nn, gx, gz, x0, x1, z0, z1, dx, dz, cfl, ko, vgrid_mode)
         if not p:
             raise ValueError("orc_create rejected the configuration")
         self._p = p
         L.orc_set_physics(p, mu, pmo, renorm)
+        if L.orc_set_boundary(p, bc, zc) != 0:
+            self.close()
+            raise ValueError("orc_set_boundary rejected the configuration")
 
     def close(self):
         if self._p:
@@ -213,8 +223,10 @@ class OracleSim:
             arr[f] = lib().orc_field_ptr(self._p, buf, f)
         return arr
 
-    def fill_ghosts(self, buf=0):
-        lib().orc_periodic_ghosts(self._p, self._ptrs(buf))
+    def fill_ghosts(self, buf=0, t=None):
+        """Boundary fill of the configured mode (open z: at time t, default
+        the current physical time)."""
+        lib().orc_fill_ghosts(self._p, self._ptrs(buf), self.phy_time if t is None else t)
 
     def ghosted(self, buf=0):
         """(8, nx+2gx, nz+2gz, nv) view of a buffer including its ghost band."""
@@ -223,7 +235,8 @@ class OracleSim:
                          for f in range(8)])
 
     def calrhs(self, state):
-        """RHS of an interior state (8,nx,nz,nv): periodic ghosts then calRHS."""
+        """RHS of an interior state (8,nx,nz,nv): boundary fill at the current
+        time, then calRHS."""
         self.set_state(state, buf=self.BUF_PRE)
         self.fill_ghosts(self.BUF_PRE)
         lib().orc_calrhs(self._p, self._ptrs(self.BUF_COR), self._ptrs(self.BUF_PRE))

@@ -92,3 +92,37 @@ def test_autotune_keeps_state_and_results(nuosc_mod):
         g.set_zlen(zl)
         g.step_rk4(6)
         assert np.array_equal(g.get_state(), want), zl
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_batch_zlen_and_autotune(nuosc_mod, precision):
+    """The batch's shared z-segment length (and its autotuner) changes only
+    the decomposition: states stay untouched by tuning and every length steps
+    every member to the bits it reaches alone; an open-boundary run is
+    refused."""
+    N = nuosc_mod
+    alone = [make(N, c, 200 + k, precision) for k, c in enumerate(RUNS)]
+    for g in alone:
+        g.step_rk4(5)
+    for zl in (0, 4, 64, None):
+        mixed = [make(N, c, 200 + k, precision) for k, c in enumerate(RUNS)]
+        b = N.Batch(precision=precision)
+        for g in mixed:
+            b.add(g)
+        if zl is None:
+            before = [g.get_state() for g in mixed]
+            best, ms = b.autotune(trials=1)
+            assert best in (16, 32, 64, 128, 256) and ms > 0
+            for g, s in zip(mixed, before):
+                assert np.array_equal(g.get_state(), s)
+        else:
+            b.set_zlen(zl)
+        b.step(5)
+        for a, m in zip(alone, mixed):
+            assert np.array_equal(a.get_state(), m.get_state()), zl
+        b.close()
+    op = N.NuOsc(precision=precision, bc=N.BC_OPEN_Z, zc=-8.0, **RUNS[3])
+    op.fillInitValue(ipt=0, alpha=0.9, eps0=1e-3, sigma=1.0, lnue=0.6, lnueb=0.5)
+    b = N.Batch(precision=precision)
+    with pytest.raises(N.NuOscError):
+        b.add(op)

@@ -86,3 +86,37 @@ def test_cli_unknown_flag(tmp_path):
     p = run_cli(tmp_path, ["--bogus", "1"])
     assert p.returncode == 0
     assert "Unreconganized parameters --bogus!" in p.stdout
+
+
+def test_cli_open_boundary_and_growth_file(tmp_path, oracle_mod):
+    """--bc open --zc: the 1-D open column with an entering pulse; analysis.dat
+    against the oracle and growth.dat holding |rho_ex| per analysis call."""
+    # one x cell: --xmax after --zmax, --dx after --dz (which sets dx too)
+    args = ["--zmax", "6.4", "--xmax", "0.05", "--dz", "0.1", "--dx", "0.1", "--nv", "2", "--mu", "1", "--pmo", "0",
+            "--ko", "0.1", "--cfl", "0.4", "--ipt", "0", "--eps0", "1e-3", "--sigma", "1", "--alpha", "0.9",
+            "--lnue", "0.6", "--lnueb", "0.5", "--END_STEP", "40", "--ANA_EVERY", "20", "--bc", "open",
+            "--zc", "-8"]
+    p = run_cli(tmp_path, args)
+    assert p.returncode == 0, p.stderr
+    assert "z open" in p.stdout
+    rows = [l for l in open(tmp_path / "analysis.dat").read().splitlines()[1:] if l.strip()]
+    grow = [l for l in open(tmp_path / "growth.dat").read().splitlines()[1:] if l.strip()]
+    assert len(rows) == len(grow) == 3
+    o = oracle_mod.OracleSim(nn=2, x0=-0.05, x1=0.05, z0=-6.4, z1=6.4, dx=0.1, dz=0.1, cfl=0.4, ko=0.1, mu=1.0,
+                             pmo=0.0, bc=1, zc=-8.0)
+    assert o.shape == (1, 128, 42)
+    o.fill_init(ipt=0, alpha=0.9, eps0=1e-3, sigma=1.0, lnue=0.6, lnueb=0.5)
+    for k in range(3):
+        if k:
+            o.step(20)
+        out, _ = o.analysis()
+        exp = o.analysis_line(out).split()
+        got = rows[k].split()
+        for a, b in zip(got[1:], exp[1:]):
+            a, b = float(a), float(b)
+            assert abs(a - b) <= 1e-10 * abs(b) + 1e-13, (rows[k], exp)
+        st = o.get_state()
+        rex = np.sqrt(o.s.ds_L * np.sum(o.vgrid()[3][None, None, :] * (st[2] ** 2 + st[3] ** 2)))
+        t, r = map(float, grow[k].split())
+        assert t == float(exp[0]) or abs(t - float(exp[0])) <= 1e-12
+        assert abs(r - rex) <= 1e-9 * rex
