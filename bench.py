@@ -50,6 +50,9 @@ def parse():
                    help="box: one large run (default); batched: 64 uneven runs stepped together vs one by one; "
                         "served: Poisson arrivals into the run server")
     p.add_argument("--runs", type=int, default=64)
+    p.add_argument("--grids", default=None,
+                   help="batched/served: comma-separated square grid sizes the runs are drawn from "
+                        "(default 64,128,192,256 batched; 64,...,384 served)")
     p.add_argument("--rate", type=float, default=2.0, help="served: arrivals per second")
     p.add_argument("--duration", type=float, default=60.0, help="served: seconds of arrivals")
     p.add_argument("--copy-calibration", action="store_true",
@@ -106,15 +109,17 @@ def bench_batched(a):
     import nuosc_amd as N
     prec = N.PREC_FP64 if a.precision == 64 else N.PREC_FP32
     rng = np.random.default_rng(7)
-    specs = [scan_run(rng, k) for k in range(a.runs)]
+    grids = tuple(int(x) for x in (a.grids or "64,128,192,256").split(","))
+    specs = [scan_run(rng, k, grids=grids) for k in range(a.runs)]
     sims = []
     for geo, init in specs:
         g = N.NuOsc(device=0, precision=prec, use_graph=True, **geo)
         g.fillInitValue(**init)
         sims.append(g)
     upd = sum(g.nx * g.nz * g.nv for g in sims)
-    # one by one
+    # one by one, each run with its own autotuned z-segment length
     for g in sims:
+        g.autotune(trials=2)
         g.step_rk4(1)
         g.synchronize()
     t0 = time.perf_counter()
@@ -126,6 +131,7 @@ def bench_batched(a):
     b = N.Batch(device=0, precision=prec)
     for g in sims:
         b.add(g)
+    zbest, _ = b.autotune(trials=2)
     b.step(1)
     t0 = time.perf_counter()
     b.step(a.steps)
@@ -139,8 +145,8 @@ def bench_batched(a):
         "speedup_vs_sequential": t_seq / t_bat,
         "n_gpus": 1, "steps": a.steps, "runs": a.runs,
         "dtype": "f64" if a.precision == 64 else "f32",
-        "config": {"workload": f"{a.runs} runs, grids in 64^2..256^2, 162/642 directions, uneven physics",
-                   "updates_per_step": upd},
+        "config": {"workload": f"{a.runs} runs, grids drawn from {list(grids)} squared, 162/642 directions, "
+                               f"uneven physics", "updates_per_step": upd, "batch_zlen": zbest},
         "state_ok": ok,
     }
     b.close()
@@ -156,6 +162,7 @@ def bench_served(a):
     import nuosc_amd as N
     prec = N.PREC_FP64 if a.precision == 64 else N.PREC_FP32
     rng = np.random.default_rng(11)
+    grids = tuple(int(x) for x in (a.grids or "64,128,192,256,320,384").split(","))
     srv = N.Server(device=0, precision=prec, hbm_budget_bytes=200 << 30, max_tick_steps=10)
     t_start = time.perf_counter()
     n, k = 0, 0
@@ -166,7 +173,7 @@ def bench_served(a):
         if now < next_t:
             time.sleep(min(0.05, next_t - now))
             continue
-        geo, init = scan_run(rng, k, grids=(64, 128, 192, 256, 320, 384))
+        geo, init = scan_run(rng, k, grids=grids)
         nsteps = int(rng.integers(a.steps, 10 * a.steps + 1))
         rid = srv.submit(nsteps=nsteps, ana_every=max(1, nsteps // 20), **geo, **init)
         submitted[rid] = nsteps
@@ -191,7 +198,8 @@ def bench_served(a):
         "queue_p50_s": float(np.percentile(queue_wait, 50)), "queue_p99_s": float(np.percentile(queue_wait, 99)),
         "gpu_busy_frac": st.busy_s / wall,
         "dtype": "f64" if a.precision == 64 else "f32",
-        "config": {"workload": f"Poisson {a.rate}/s for {a.duration}s, grids 64^2..384^2, 162/642 directions, "
+        "config": {"workload": f"Poisson {a.rate}/s for {a.duration}s, grids drawn from {list(grids)} squared, "
+                               f"162/642 directions, "
                                f"steps U[{a.steps},{10 * a.steps}], 200 GB budget"},
     }
     print(json.dumps(out), flush=True)
@@ -253,10 +261,11 @@ def main():
     nx, nz, nv = g.shape
     upd = nx * nz * nv
 
-    # per-stage kernel durations from HIP events on the library's stream
-    stage_ms = g.profile_stages(2)
     g.step_rk4(a.warmup)
     g.synchronize()
+    # per-stage kernel durations from HIP events around each stage kernel on
+    # the library's stream, right before the timed steps
+    stage_ms = g.profile_stages(3)
     t0 = time.perf_counter()
     g.step_rk4(a.steps)
     g.synchronize()

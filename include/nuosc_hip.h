@@ -189,10 +189,10 @@ int nuosc_p3_slice(nuosc_sim* sim, const int* xidx, int sx, const int* vidx, int
  * from HIP events on the handle's stream (synchronises). */
 int nuosc_last_step_ms(nuosc_sim* sim, double* ms);
 
-/* Measurement helper: run nsteps RK4 steps eagerly with HIP events between
- * the four stage launches (on the handle's stream) and return the average
- * duration of each stage kernel in stage_ms[4] (ms).  Advances the state and
- * phy_time like nuosc_step_rk4. */
+/* Measurement helper: run nsteps RK4 steps eagerly with a pair of HIP events
+ * around each of the four stage kernels (on the handle's stream) and return
+ * the average duration of each stage kernel in stage_ms[4] (ms).  Advances
+ * the state and phy_time like nuosc_step_rk4. */
 int nuosc_profile_stages(nuosc_sim* sim, int nsteps, double* stage_ms);
 
 /* Stage-kernel decomposition (no reference counterpart).  Each wavefront

@@ -112,7 +112,8 @@ int ensure_tmp(nuosc_sim* s, size_t bytes) {
 }
 
 // One RK4 step: the four stage launches (+ optional renormalisation).  With
-// ev != nullptr an event is recorded before each stage and after the last.
+// ev != nullptr events ev[2k], ev[2k+1] bracket the stage-(k+1) kernel alone
+// (not its boundary fill or moment-total pass).
 template <typename T>
 int enqueue_step(nuosc_sim* s, hipEvent_t* ev = nullptr) {
     const double dt = s->dt;
@@ -131,24 +132,28 @@ int enqueue_step(nuosc_sim* s, hipEvent_t* ev = nullptr) {
         return open ? launch_open_ghosts<T>(g, y, s->d_g0, s->d_g0 + s->nv, s->inflow, s->d_time, cdt, s->stream)
                     : hipSuccess;
     };
-    if (ev) HIPCK(s, hipEventRecord(ev[0], s->stream));
+    auto mark = [&](int k) { return ev ? hipEventRecord(ev[k], s->stream) : hipSuccess; };
     HIPCK(s, ghosts(u, 0.0));
+    HIPCK(s, mark(0));
     HIPCK(s, launch_stage<T>(1, g, s->ph, s->plan, u, nullptr, w, ya, s->mom_u, P, c6, c2, s->stream));
+    HIPCK(s, mark(1));
     HIPCK(s, launch_moment_totals(g, comp, P, s->mom_a, s->stream));
-    if (ev) HIPCK(s, hipEventRecord(ev[1], s->stream));
     HIPCK(s, ghosts(ya, c2));
+    HIPCK(s, mark(2));
     HIPCK(s, launch_stage<T>(2, g, s->ph, s->plan, ya, u, w, yb, s->mom_a, P, c3, c2, s->stream));
+    HIPCK(s, mark(3));
     HIPCK(s, launch_moment_totals(g, comp, P, s->mom_b, s->stream));
-    if (ev) HIPCK(s, hipEventRecord(ev[2], s->stream));
     HIPCK(s, ghosts(yb, c2));
+    HIPCK(s, mark(4));
     HIPCK(s, launch_stage<T>(3, g, s->ph, s->plan, yb, u, w, ya, s->mom_b, P, c3, dt, s->stream));
+    HIPCK(s, mark(5));
     HIPCK(s, launch_moment_totals(g, comp, P, s->mom_a, s->stream));
-    if (ev) HIPCK(s, hipEventRecord(ev[3], s->stream));
     HIPCK(s, ghosts(ya, dt));
+    HIPCK(s, mark(6));
     HIPCK(s, launch_stage<T>(4, g, s->ph, s->plan, ya, nullptr, w, u, s->mom_a, P, c6, 0.0, s->stream));
+    HIPCK(s, mark(7));
     HIPCK(s, launch_moment_totals(g, comp, P, s->mom_u, s->stream));
     if (open) HIPCK(s, launch_advance_time(s->d_time, dt, s->stream));
-    if (ev) HIPCK(s, hipEventRecord(ev[4], s->stream));
     if (s->cfg.renorm) {
         HIPCK(s, launch_renorm<T>(g, u, s->d_g0, s->d_g0 + s->nv, s->stream));
         HIPCK(s, launch_moments<T>(g, u, P, s->stream));
@@ -275,7 +280,7 @@ int nuosc_create(const nuosc_config* cfg, nuosc_sim** out) {
     s->nvp = (s->nv + align - 1) / align * align;
     const int lpr = kWave / rows_per_wave((int)s->esize);
     s->nch = (s->nv + lpr - 1) / lpr;
-    s->fs = (long long)s->nx * nzs * s->nvp;
+    s->fs = (long long)(s->nx + 1) * nzs * s->nvp;   // + the stage kernels' spare x row
     if (s->fs * (long long)s->esize >= (1ll << 32) - 1) {
         delete s;
         return fail(nullptr, NUOSC_EINVAL, "one field must stay below 4 GiB (nx*nz*nv too large for one GPU handle)");
@@ -300,7 +305,7 @@ int nuosc_create(const nuosc_config* cfg, nuosc_sim** out) {
         return bail(NUOSC_EHIP, "event creation failed");
 
     const size_t state_bytes = (size_t)kNVar * s->fs * s->esize;
-    const size_t part_bytes = (size_t)s->nx * s->nz * s->nch * kMomRec * sizeof(double);
+    const size_t part_bytes = (size_t)(s->nx + 1) * s->nz * s->nch * kMomRec * sizeof(double);
     const size_t mom_bytes = (size_t)s->nx * s->nz * kMomStride * sizeof(double);
     void** states[4] = {&s->st_u, &s->st_w, &s->st_ya, &s->st_yb};
     for (void** p : states) {
@@ -569,7 +574,7 @@ int nuosc_profile_stages(nuosc_sim* s, int nsteps, double* stage_ms) {
         int rc = refresh_moments(s);
         if (rc) return rc;
     }
-    hipEvent_t ev[5] = {};
+    hipEvent_t ev[8] = {};
     for (auto& e : ev)
         if (hipEventCreate(&e) != hipSuccess) return fail(s, NUOSC_EHIP, "event creation failed");
     double acc[4] = {0, 0, 0, 0};
@@ -578,13 +583,13 @@ int nuosc_profile_stages(nuosc_sim* s, int nsteps, double* stage_ms) {
         rc = is64(s) ? enqueue_step<double>(s, ev) : enqueue_step<float>(s, ev);
         if (rc) break;
         s->phy_time += s->dt;
-        if (hipEventSynchronize(ev[4]) != hipSuccess) {
+        if (hipEventSynchronize(ev[7]) != hipSuccess) {
             rc = fail(s, NUOSC_EHIP, "event sync failed");
             break;
         }
         for (int k = 0; k < 4; ++k) {
             float f = 0;
-            hipEventElapsedTime(&f, ev[k], ev[k + 1]);
+            hipEventElapsedTime(&f, ev[2 * k], ev[2 * k + 1]);
             acc[k] += f;
         }
     }
@@ -1051,8 +1056,8 @@ int nuosc_estimate_device_bytes(const nuosc_config* c, long long* bytes) {
     const long long align = 128 / esize, nvp = (nv + align - 1) / align * align;
     const long long lpr = kWave / rows_per_wave(esize), nch = (nv + lpr - 1) / lpr;
     const long long nzs = nz + (c->bc == NUOSC_BC_OPEN_Z ? 2 * c->gz : 0);
-    long long b = 4ll * kNVar * nx * nzs * nvp * esize;         // u, w, yA, yB
-    b += nx * nz * nch * kMomRec * 8 + 3ll * nx * nz * kMomStride * 8;  // moments
+    long long b = 4ll * kNVar * (nx + 1) * nzs * nvp * esize;   // u, w, yA, yB (+ spare x row)
+    b += (nx + 1) * nz * nch * kMomRec * 8 + 3ll * nx * nz * kMomStride * 8;  // moments
     b += 4 * nv * 8 + 2 * nv * 8 + 2048ll * kNAna * 16 + kNAna * 8;    // v-grid, G0, analysis
     b += 2 * nz * 8;                                                   // init tables
     *bytes = b;

@@ -59,10 +59,12 @@ __device__ __forceinline__ int xcd_remap(int b, int nb) {
 // Orders the LDS accesses of a wavefront: everything before is visible to
 // every lane of the wave after (the wave executes in lockstep; this keeps the
 // compiler from moving LDS accesses across it).
+// The fences cover LDS only, so outstanding global loads (prefetches for
+// later z steps) are not waited for here.
 __device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
     __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
 }
 
 // Knuth/Dekker two-sum: s + e == a + b exactly.
@@ -161,13 +163,14 @@ typedef const __attribute__((address_space(4))) double cdouble;
 //
 // A wavefront covers R adjacent x-rows x L = 64/R directions (FP64: 4 x 16,
 // FP32: 2 x 32, so every row segment is one 128-byte line) and marches along
-// z over one segment.  Per lane it keeps a 5-deep z-window of its own row and
-// a 3-deep FIFO of the halo rows (i0-2, i0-1, i0+R, i0+R+1), both for all 8
-// fields.  Halo rows are loaded at z+2, in the same iteration in which the
-// neighbouring waves load those rows as their own window rows, so the second
-// read hits in L2 instead of HBM.  The x-neighbours are exchanged through a
-// wave-private LDS slab (no barrier: one wave's LDS operations complete in
-// order).
+// z over one segment.  Per lane it keeps a 5-deep z-window of its own row for
+// all 8 fields in registers.  The halo rows (i0-2, i0-1, i0+R, i0+R+1) are
+// loaded at z+2, in the same step in which the neighbouring waves of the
+// block load those rows as their own window rows (so the second read hits in
+// L2), and staged, with the own rows, into a wave-private two-level LDS slab
+// through which the x-neighbours are read (no block barrier: one wave's LDS
+// operations complete in order).  Moments of z+1 are prefetched during step
+// z.
 // ---------------------------------------------------------------------------
 template <typename T, int FPD>
 struct StageArgs {
@@ -175,7 +178,8 @@ struct StageArgs {
     Fields<T, FPD> yin, u, w, yout;
     unsigned long long mtot;  // moment totals of yin: [site][kMomStride] doubles
     unsigned mtot_bytes;
-    double* mpart;            // partial moments of yout: [site][nch][kMomRec]
+    unsigned long long mpart; // partial moments of yout: [site][nch][kMomRec] (+ one spare x row)
+    unsigned mpart_bytes;
     double cw, cy;
     double mu2, mu, p2st, p2ct, pmo, ct2, st;
     int zlen, nquads;
@@ -286,6 +290,9 @@ __device__ __forceinline__ void stage_body(const StageArgs<T, FPD>& a, int lbloc
     int rmin = wrapi(i0 - 2, nx);
     rmin = min(rmin, wrapi(i0 - 1, nx));
     for (int q = 0; q < R + 2; ++q) rmin = min(rmin, wrapi(i0 + q, nx));
+    // integer division leaves uniform values in vector registers; the buffer
+    // offsets derived from rmin must sit in scalar registers
+    rmin = __builtin_amdgcn_readfirstlane(rmin);
     const unsigned rbase = (unsigned)rmin * nzs + zoff;   // stored row of (rmin, z = 0)
     const unsigned mbase = (unsigned)rmin * nz;           // moment record of (rmin, z = 0)
     const unsigned fieldb = (unsigned)nzs * rowb;  // bytes per x-row of one field
@@ -293,6 +300,12 @@ __device__ __forceinline__ void stage_body(const StageArgs<T, FPD>& a, int lbloc
     // ghost rows hold jj in [-zoff, nz + zoff))
     auto zrow = [&](int jj) { return (unsigned)(wrapi(jj + zoff, nzs) - zoff); };
     const unsigned vo = (unsigned)(rw - rmin) * fieldb + (unsigned)vl * sizeof(T);
+    // Stores are unconditional (no branch, so the compiler can count the
+    // outstanding memory operations across iterations exactly): a lane past
+    // the last x row writes the spare row nx of every field, a lane past nv
+    // writes its own padding column; neither is ever read.
+    const int rst = row_ok ? rw : nx;
+    const unsigned vos = (unsigned)(rst - rmin) * fieldb + (unsigned)v * sizeof(T);
     unsigned vh[NH];
     int slot_h[NH];
 #pragma unroll
@@ -320,46 +333,54 @@ __device__ __forceinline__ void stage_body(const StageArgs<T, FPD>& a, int lbloc
     const int j1 = min(nz, j0 + a.zlen);
 
     T win[kNVar][5];   // own row, z = j-2 .. j+2
-    T hreg[NH][kNVar]; // halo rows loaded for z = j+2, staged into LDS one iteration later
+    T hreg[NH][kNVar]; // halo rows of z = j+2, staged into LDS at the end of step j
+    // velocity moments of the stage input at this lane's site, z = j (M) and
+    // z = j+1 (Mn, in flight during step j)
+    double M[kNMom], Mn[kNMom];
+    unsigned long long bm = a.mtot;
+    asm volatile("" : "+s"(bm));
+    const __amdgpu_buffer_rsrc_t rm = __builtin_amdgcn_make_buffer_rsrc((void*)bm, 0, (int)a.mtot_bytes, kRsrcWord3);
+    auto load_moments = [&](double (&dst)[kNMom], int jj) {
+        const unsigned sm = (mbase + jj) * (kMomStride * 8);
+#pragma unroll
+        for (int s = 0; s < kNMom; s += 2) {
+            const u32x4 x = __builtin_amdgcn_raw_buffer_load_b128(rm, vom + s * 8, sm, 0);
+            dst[s] = __builtin_bit_cast(double, u32x2{x[0], x[1]});
+            dst[s + 1] = __builtin_bit_cast(double, u32x2{x[2], x[3]});
+        }
+    };
 #pragma unroll
     for (int qz = 0; qz < 4; ++qz) {
         const unsigned so = (rbase + zrow(j0 - 2 + qz)) * rowb;
 #pragma unroll
         for (int f = 0; f < kNVar; ++f) win[f][qz] = a.yin.ld(a.yin.base, f, vo, so);
     }
-    {
-        const unsigned so = (rbase + zrow(j0)) * rowb;
+    // halo rows of z = j0 and j0+1 straight into their levels
+#pragma unroll
+    for (int qz = 0; qz < 2; ++qz) {
+        const unsigned so = (rbase + zrow(j0 + qz)) * rowb;
 #pragma unroll
         for (int h = 0; h < NH; ++h)
 #pragma unroll
-            for (int f = 0; f < kNVar; ++f) xs[j0 & 1][f][slot_h[h]] = a.yin.ld(a.yin.base, f, vh[h], so);
-        const unsigned s1 = (rbase + zrow(j0 + 1)) * rowb;
-#pragma unroll
-        for (int h = 0; h < NH; ++h)
-#pragma unroll
-            for (int f = 0; f < kNVar; ++f) hreg[h][f] = a.yin.ld(a.yin.base, f, vh[h], s1);
+            for (int f = 0; f < kNVar; ++f)
+                xs[(j0 + qz) & 1][f][slot_h[h]] = a.yin.ld(a.yin.base, f, vh[h], so);
     }
+    load_moments(M, j0);
 
     for (int j = j0; j < j1; ++j) {
         const unsigned so = (rbase + j) * rowb;
         const int lv = j & 1;
         wave_sync();   // the previous iteration's LDS reads stay before these writes
-        // halo rows of z = j+1 (loaded one iteration ago) into the other level
+        // own rows of z = j into this level (the halo rows of z = j are there)
 #pragma unroll
-        for (int h = 0; h < NH; ++h)
-#pragma unroll
-            for (int f = 0; f < kNVar; ++f) xs[lv ^ 1][f][slot_h[h]] = hreg[h][f];
-        // ---- all loads of the iteration
+        for (int f = 0; f < kNVar; ++f) xs[lv][f][slot_c] = win[f][2];
+        // ---- all loads of the iteration, in the order they are consumed:
+        // own row z+2 (stencil), u and w (update), halo rows z+2 (next
+        // iteration), moments of z+1 (next iteration)
         const unsigned long long by = a.yin.fresh_base();
-        {
-            const unsigned s2 = (rbase + zrow(j + 2)) * rowb;
+        const unsigned s2 = (rbase + zrow(j + 2)) * rowb;
 #pragma unroll
-            for (int f = 0; f < kNVar; ++f) win[f][4] = a.yin.ld(by, f, vo, s2);
-#pragma unroll
-            for (int h = 0; h < NH; ++h)
-#pragma unroll
-                for (int f = 0; f < kNVar; ++f) hreg[h][f] = a.yin.ld(by, f, vh[h], s2);
-        }
+        for (int f = 0; f < kNVar; ++f) win[f][4] = a.yin.ld(by, f, vo, s2);
         T uc[kNVar], wc[kNVar];
         if constexpr (STAGE == 2 || STAGE == 3) {
             const unsigned long long bu = a.u.fresh_base();
@@ -371,24 +392,13 @@ __device__ __forceinline__ void stage_body(const StageArgs<T, FPD>& a, int lbloc
 #pragma unroll
             for (int f = 0; f < kNVar; ++f) wc[f] = a.w.ld(bw, f, vo, so);
         }
-        // velocity moments of the stage input at this lane's site
-        double M[kNMom];
-        {
-            unsigned long long bm = a.mtot;
-            asm volatile("" : "+s"(bm));
-            const __amdgpu_buffer_rsrc_t rm =
-                __builtin_amdgcn_make_buffer_rsrc((void*)bm, 0, (int)a.mtot_bytes, kRsrcWord3);
-            const unsigned sm = (mbase + j) * (kMomStride * 8);
 #pragma unroll
-            for (int s = 0; s < kNMom; s += 2) {
-                const u32x4 x = __builtin_amdgcn_raw_buffer_load_b128(rm, vom + s * 8, sm, 0);
-                M[s] = __builtin_bit_cast(double, u32x2{x[0], x[1]});
-                M[s + 1] = __builtin_bit_cast(double, u32x2{x[2], x[3]});
-            }
-        }
-        // ---- own rows of z = j into this level; halo rows of z = j are there
+        for (int h = 0; h < NH; ++h)
 #pragma unroll
-        for (int f = 0; f < kNVar; ++f) xs[lv][f][slot_c] = win[f][2];
+            for (int f = 0; f < kNVar; ++f) hreg[h][f] = a.yin.ld(by, f, vh[h], s2);
+        // unconditional (a branch here would make every later wait a full
+        // drain); the last row of the box reloads itself
+        load_moments(Mn, min(j + 1, nz - 1));
         wave_sync();   // other lanes' writes are visible to the reads below
         // projections onto direction v (INP, nuosc_class.cpp:187)
         const T A = (T)(M[0] - M[1] * vxd - M[2] * vyd - M[3] * vzd);
@@ -437,23 +447,30 @@ __device__ __forceinline__ void stage_body(const StageArgs<T, FPD>& a, int lbloc
                 yo[f] = wc[f] + cw * k[f];
             }
         }
-        if (act) {
+        {
             const unsigned long long bw = a.w.fresh_base();
             const unsigned long long bo = a.yout.fresh_base();
 #pragma unroll
             for (int f = 0; f < kNVar; ++f) {
-                if constexpr (STAGE >= 1 && STAGE <= 3) a.w.st(bw, f, wn[f], vo, so);
-                a.yout.st(bo, f, yo[f], vo, so);
+                if constexpr (STAGE >= 1 && STAGE <= 3) a.w.st(bw, f, wn[f], vos, so);
+                a.yout.st(bo, f, yo[f], vos, so);
             }
         }
         if constexpr (STAGE != 0) {
             double t[16];
             moment_terms<T>(yo, q, t);
             const double tot = group_sum16<L, COMP>(t, lane);
+            // lane dl of a row holds value dl >> sh; every lane stores it
+            // (lanes sharing a value store the same bits; values 12..15 fill
+            // unused slots of the 16-double record)
             constexpr int sh = (L == 16) ? 0 : (L == 32 ? 1 : 2);
-            const int m = dl >> sh;
-            if (row_ok && (dl & ((1 << sh) - 1)) == 0 && m < kNMom)
-                a.mpart[((long long)(rw * nz + j) * nch + ch) * kMomRec + m] = tot;
+            unsigned long long bp = a.mpart;
+            asm volatile("" : "+s"(bp));
+            const __amdgpu_buffer_rsrc_t rp =
+                __builtin_amdgcn_make_buffer_rsrc((void*)bp, 0, (int)a.mpart_bytes, kRsrcWord3);
+            const unsigned rec = ((unsigned)(rst - rmin) * (unsigned)nz * nch + (unsigned)ch) * (kMomRec * 8);
+            const unsigned srec = (mbase + j) * (unsigned)(nch * kMomRec * 8);
+            Lane<double>::st(tot, rp, rec + (unsigned)(dl >> sh) * 8, srec);
         }
 #pragma unroll
         for (int f = 0; f < kNVar; ++f) {
@@ -462,11 +479,23 @@ __device__ __forceinline__ void stage_body(const StageArgs<T, FPD>& a, int lbloc
             win[f][2] = win[f][3];
             win[f][3] = win[f][4];
         }
+#pragma unroll
+        for (int q = 0; q < kNMom; ++q) M[q] = Mn[q];
+        // halo rows of z = j+2 (loaded in this iteration) into this level,
+        // whose z = j reads are done; level lv ^ 1 holds z = j+1.  Done here
+        // rather than at the top of the next iteration: a wait at the loop
+        // head for a load of the previous iteration becomes a full drain of
+        // the memory counter.
+        wave_sync();
+#pragma unroll
+        for (int h = 0; h < NH; ++h)
+#pragma unroll
+            for (int f = 0; f < kNVar; ++f) xs[lv][f][slot_h[h]] = hreg[h][f];
     }
 }
 
 template <typename T, int R, int FPD, int STAGE, bool COMP>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_stage(const StageArgs<T, FPD> a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 2))) void k_stage(const StageArgs<T, FPD> a) {
     __shared__ Slab<T, R> slab[4];
     const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     stage_body<T, R, FPD, STAGE, COMP>(a, xcd_remap(blockIdx.x, gridDim.x), wib, slab[wib]);
@@ -475,7 +504,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 // Many independent runs in one launch: logical block -> (run, block of run)
 // from bmap; each run's arguments from the runs table (read with scalar loads).
 template <typename T, int R, int FPD, int STAGE, bool COMP>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_stage_batch(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 2))) void k_stage_batch(
     const StageArgs<T, FPD>* __restrict__ runs, const int* __restrict__ bmap) {
     __shared__ Slab<T, R> slab[4];
     const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -553,7 +582,8 @@ static StageArgs<T, FPD> make_stage_args(const Geometry& g, const Physics& ph, c
     a.yout = fields_of<T, FPD>(g, yout);
     a.mtot = (unsigned long long)mtot;
     a.mtot_bytes = (unsigned)((long long)g.nx * g.nz * kMomStride * sizeof(double));
-    a.mpart = mpart;
+    a.mpart = (unsigned long long)mpart;
+    a.mpart_bytes = (unsigned)((long long)(g.nx + 1) * g.nz * g.nch * kMomRec * sizeof(double));
     a.cw = cw;
     a.cy = cy;
     a.mu2 = 2 * ph.mu;
