@@ -168,8 +168,12 @@ def bench_served(a):
     n, k = 0, 0
     next_t = rng.exponential(1.0 / a.rate)
     submitted = {}
+    last_note = 0.0
     while next_t < a.duration:
         now = time.perf_counter() - t_start
+        if now - last_note > 30:
+            print(f"served: {k} requests submitted at {now:.1f} s", file=sys.stderr, flush=True)
+            last_note = now
         if now < next_t:
             time.sleep(min(0.05, next_t - now))
             continue
@@ -179,13 +183,18 @@ def bench_served(a):
         submitted[rid] = nsteps
         k += 1
         next_t += rng.exponential(1.0 / a.rate)
+    print(f"served: {len(submitted)} requests submitted in {time.perf_counter() - t_start:.1f} s", file=sys.stderr,
+          flush=True)
     lat, queue_wait, bad = [], [], 0
-    for _ in submitted:
+    for n_done in range(1, len(submitted) + 1):
         r = srv.take(timeout=3600)
         info, series = r
         bad += info.status != 0
         lat.append(info.t_done - info.t_submit)
         queue_wait.append(info.t_admit - info.t_submit)
+        if n_done % 10 == 0:
+            print(f"served: {n_done}/{len(submitted)} done at {time.perf_counter() - t_start:.1f} s", file=sys.stderr,
+                  flush=True)
     st = srv.stats()
     srv.close()
     wall = time.perf_counter() - t_start

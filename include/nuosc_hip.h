@@ -152,6 +152,11 @@ int nuosc_fill_init(nuosc_sim* sim, int ipt, double alpha, double eps0, double s
 /* NuOsc::step_rk4 (nuosc_class.cpp:374-427), nsteps times; phy_time += dt
  * per step.  Asynchronous on the handle's stream. */
 int nuosc_step_rk4(nuosc_sim* sim, int nsteps);
+/* NuOsc::step_srk3 (nuosc_class.cpp:429-452), the 3-stage SSP Runge-Kutta
+ * step, nsteps times; 8 state passes per step instead of RK4's 16.  Unlike
+ * the reference's step_srk3, which refreshes no ghost cells, every stage
+ * input gets the boundary fill (stage times t, t+dt, t+dt/2). */
+int nuosc_step_srk3(nuosc_sim* sim, int nsteps);
 int nuosc_synchronize(nuosc_sim* sim);
 
 /* NuOsc::calRHS(out, in) (nuosc_class.cpp:11-307) with the boundary fill of

@@ -110,6 +110,7 @@ SIGNATURES = {
     "nuosc_get_device_bytes": ([_vp, C.POINTER(C.c_longlong)], C.c_int),
     "nuosc_fill_init": ([_vp, C.c_int] + [C.c_double] * 7 + [C.c_uint64], C.c_int),
     "nuosc_step_rk4": ([_vp, C.c_int], C.c_int),
+    "nuosc_step_srk3": ([_vp, C.c_int], C.c_int),
     "nuosc_synchronize": ([_vp], C.c_int),
     "nuosc_calrhs": ([_vp, _dp, _dp], C.c_int),
     "nuosc_analysis": ([_vp, C.POINTER(Analysis)], C.c_int),
@@ -287,6 +288,10 @@ class NuOsc:
 
     def step_rk4(self, nsteps=1):
         self._check(lib().nuosc_step_rk4(self._h, int(nsteps)))
+
+    def step_srk3(self, nsteps=1):
+        """SSP-RK3 steps (NuOsc::step_srk3, with boundary fills per stage)."""
+        self._check(lib().nuosc_step_srk3(self._h, int(nsteps)))
 
     def synchronize(self):
         self._check(lib().nuosc_synchronize(self._h))

@@ -153,6 +153,13 @@ void NuOsc::step_rk4(int nsteps) {
     nuosc_get_scalars(sim_, nullptr, nullptr, &phy_time, nullptr);
 }
 
+void NuOsc::step_srk3() { step_srk3(1); }
+
+void NuOsc::step_srk3(int nsteps) {
+    check(nuosc_step_srk3(sim_, nsteps), "step_srk3");
+    nuosc_get_scalars(sim_, nullptr, nullptr, &phy_time, nullptr);
+}
+
 void NuOsc::synchronize() { check(nuosc_synchronize(sim_), "synchronize"); }
 
 void NuOsc::calRHS(std::vector<double>& out, const std::vector<double>& in) {

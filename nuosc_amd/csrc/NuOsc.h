@@ -3,7 +3,7 @@
 // include/nuosc_hip.h.  The state lives on the GPU; host code sees copies.
 //
 // Kept: constructor arguments, set_mu / set_pmo / set_renorm, get_lpts /
-// get_nv, fillInitValue, step_rk4, calRHS, analysis (stdout line +
+// get_nv, fillInitValue, step_rk4, step_srk3, calRHS, analysis (stdout line +
 // analysis.dat), renormalize, addSnapShotAtV / addSnapShotAtXV / checkSnapShot
 // and the public scalars phy_time, dt, dx, dz, ds_L, CFL, ko, pmo, mu, renorm,
 // n_nue0, theta/ct/st.
@@ -75,6 +75,8 @@ public:
                        double lnuebx, uint64_t seed = 0);
     void step_rk4();
     void step_rk4(int nsteps);
+    void step_srk3();
+    void step_srk3(int nsteps);
     void synchronize();
     // out/in: 8 fields in the interior layout [nx][nz][nv]
     void calRHS(std::vector<double>& out, const std::vector<double>& in);

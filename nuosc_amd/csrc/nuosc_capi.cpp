@@ -59,6 +59,8 @@ struct nuosc_sim {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
+    hipGraph_t graph3 = nullptr;          // SSP-RK3 step
+    hipGraphExec_t graph3_exec = nullptr;
     bool have_moments = false;
     int steps_in_last_call = 0;
     Geometry geo{};
@@ -92,8 +94,12 @@ T* field_ptr(void* base, const nuosc_sim* s, int f) {
 void drop_graph(nuosc_sim* s) {
     if (s->graph_exec) hipGraphExecDestroy(s->graph_exec);
     if (s->graph) hipGraphDestroy(s->graph);
+    if (s->graph3_exec) hipGraphExecDestroy(s->graph3_exec);
+    if (s->graph3) hipGraphDestroy(s->graph3);
     s->graph_exec = nullptr;
     s->graph = nullptr;
+    s->graph3_exec = nullptr;
+    s->graph3 = nullptr;
 }
 
 int ensure_tmp(nuosc_sim* s, size_t bytes) {
@@ -152,6 +158,43 @@ int enqueue_step(nuosc_sim* s, hipEvent_t* ev = nullptr) {
     HIPCK(s, mark(6));
     HIPCK(s, launch_stage<T>(4, g, s->ph, s->plan, ya, nullptr, w, u, s->mom_a, P, c6, 0.0, s->stream));
     HIPCK(s, mark(7));
+    HIPCK(s, launch_moment_totals(g, comp, P, s->mom_u, s->stream));
+    if (open) HIPCK(s, launch_advance_time(s->d_time, dt, s->stream));
+    if (s->cfg.renorm) {
+        HIPCK(s, launch_renorm<T>(g, u, s->d_g0, s->d_g0 + s->nv, s->stream));
+        HIPCK(s, launch_moments<T>(g, u, P, s->stream));
+        HIPCK(s, launch_moment_totals(g, comp, P, s->mom_u, s->stream));
+    }
+    return NUOSC_OK;
+}
+
+// One SSP-RK3 step (nuosc_class.cpp:429-452): u1 = u + dt L(u) -> yA,
+// u2 = 3/4 u + 1/4 (u1 + dt L(u1)) -> yB, u' = 1/3 u + 2/3 (u2 + dt L(u2)) -> u,
+// each stage input with its boundary fill (stage times t, t+dt, t+dt/2).
+template <typename T>
+int enqueue_step_srk3(nuosc_sim* s) {
+    const double dt = s->dt;
+    T* u = static_cast<T*>(s->st_u);
+    T* ya = static_cast<T*>(s->st_ya);
+    T* yb = static_cast<T*>(s->st_yb);
+    const Geometry& g = s->geo;
+    const bool comp = sizeof(T) == 4;
+    double* P = s->mom_part;
+    const bool open = g.bc == NUOSC_BC_OPEN_Z;
+    auto ghosts = [&](T* y, double cdt) {
+        return open ? launch_open_ghosts<T>(g, y, s->d_g0, s->d_g0 + s->nv, s->inflow, s->d_time, cdt, s->stream)
+                    : hipSuccess;
+    };
+    HIPCK(s, ghosts(u, 0.0));
+    HIPCK(s, launch_stage<T>(5, g, s->ph, s->plan, u, nullptr, nullptr, ya, s->mom_u, P, 0.0, dt, s->stream));
+    HIPCK(s, launch_moment_totals(g, comp, P, s->mom_a, s->stream));
+    HIPCK(s, ghosts(ya, dt));
+    HIPCK(s, launch_stage<T>(6, g, s->ph, s->plan, ya, u, nullptr, yb, s->mom_a, P, 0.0, dt, s->stream, 3.0 / 4.0,
+                             1.0 / 4.0));
+    HIPCK(s, launch_moment_totals(g, comp, P, s->mom_b, s->stream));
+    HIPCK(s, ghosts(yb, 0.5 * dt));
+    HIPCK(s, launch_stage<T>(7, g, s->ph, s->plan, yb, u, nullptr, u, s->mom_b, P, 0.0, dt, s->stream, 1.0 / 3.0,
+                             2.0 / 3.0));
     HIPCK(s, launch_moment_totals(g, comp, P, s->mom_u, s->stream));
     if (open) HIPCK(s, launch_advance_time(s->d_time, dt, s->stream));
     if (s->cfg.renorm) {
@@ -278,7 +321,7 @@ int nuosc_create(const nuosc_config* cfg, nuosc_sim** out) {
     s->esize = is64(s) ? 8 : 4;
     const int align = (int)(128 / s->esize);
     s->nvp = (s->nv + align - 1) / align * align;
-    const int lpr = kWave / rows_per_wave((int)s->esize);
+    const int lpr = dirs_per_chunk((int)s->esize);
     s->nch = (s->nv + lpr - 1) / lpr;
     s->fs = (long long)(s->nx + 1) * nzs * s->nvp;   // + the stage kernels' spare x row
     if (s->fs * (long long)s->esize >= (1ll << 32) - 1) {
@@ -546,6 +589,44 @@ int nuosc_step_rk4(nuosc_sim* s, int nsteps) {
             int rc = is64(s) ? enqueue_step<double>(s) : enqueue_step<float>(s);
             if (rc) return rc;
             s->phy_time += s->dt;  // accumulated, as nuosc_class.cpp:423
+        }
+    }
+    HIPCK(s, hipEventRecord(s->ev1, s->stream));
+    s->steps_in_last_call = nsteps;
+    return NUOSC_OK;
+}
+
+int nuosc_step_srk3(nuosc_sim* s, int nsteps) {
+    if (!s) return NUOSC_EINVAL;
+    if (nsteps < 0) return fail(s, NUOSC_EINVAL, "negative step count");
+    if (!s->have_moments) {
+        int rc = refresh_moments(s);
+        if (rc) return rc;
+    }
+    HIPCK(s, hipEventRecord(s->ev0, s->stream));
+    if (s->cfg.use_graph && nsteps > 0) {
+        if (!s->graph3_exec) {
+            HIPCK(s, hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+            int rc = is64(s) ? enqueue_step_srk3<double>(s) : enqueue_step_srk3<float>(s);
+            hipGraph_t gr = nullptr;
+            hipError_t e = hipStreamEndCapture(s->stream, &gr);
+            if (rc) {
+                if (gr) hipGraphDestroy(gr);
+                return rc;
+            }
+            if (e != hipSuccess) return fail(s, NUOSC_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
+            s->graph3 = gr;
+            HIPCK(s, hipGraphInstantiate(&s->graph3_exec, s->graph3, nullptr, nullptr, 0));
+        }
+        for (int n = 0; n < nsteps; ++n) {
+            HIPCK(s, hipGraphLaunch(s->graph3_exec, s->stream));
+            s->phy_time += s->dt;
+        }
+    } else {
+        for (int n = 0; n < nsteps; ++n) {
+            int rc = is64(s) ? enqueue_step_srk3<double>(s) : enqueue_step_srk3<float>(s);
+            if (rc) return rc;
+            s->phy_time += s->dt;  // nuosc_class.cpp:448
         }
     }
     HIPCK(s, hipEventRecord(s->ev1, s->stream));
@@ -1054,7 +1135,7 @@ int nuosc_estimate_device_bytes(const nuosc_config* c, long long* bytes) {
     const long long nv = 10ll * c->nn * c->nn + 2;
     const int esize = c->precision == NUOSC_PREC_FP32 ? 4 : 8;
     const long long align = 128 / esize, nvp = (nv + align - 1) / align * align;
-    const long long lpr = kWave / rows_per_wave(esize), nch = (nv + lpr - 1) / lpr;
+    const long long lpr = dirs_per_chunk(esize), nch = (nv + lpr - 1) / lpr;
     const long long nzs = nz + (c->bc == NUOSC_BC_OPEN_Z ? 2 * c->gz : 0);
     long long b = 4ll * kNVar * (nx + 1) * nzs * nvp * esize;   // u, w, yA, yB (+ spare x row)
     b += (nx + 1) * nz * nch * kMomRec * 8 + 3ll * nx * nz * kMomStride * 8;  // moments

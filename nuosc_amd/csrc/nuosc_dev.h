@@ -38,7 +38,7 @@ enum AnaIdx {
 
 struct Geometry {
     int nx, nz, nv, nvp;
-    int lpr;               // directions per row segment of a wavefront (64 / rows per wave)
+    int lpr;               // directions per chunk: one 128-byte row segment (dirs_per_chunk)
     int nch;               // direction chunks per site = ceil(nv / lpr)
     long long fs;          // elements per field
     double twelve_dx, twelve_dz;   // 12*dx, 12*dz (advection denominators)
@@ -64,15 +64,20 @@ struct Physics {
 //   stage 1: w = yin + cw*k,  yout = yin + cy*k   (yin is the state u)
 //   stage 2/3: w = w + cw*k,  yout = u + cy*k
 //   stage 4: yout = w + cw*k                      (yout is the new state)
+// SSP-RK3 (k = L(yin)):
+//   stage 5: yout = yin + cy*k
+//   stage 6/7: yout = ca*u + cb*(yin + cy*k)        (stage 7 may write u in place)
 // mtot holds the moment totals of yin; mpart (stages 1-4) receives the partial
 // moments of yout.
 struct StagePlan {
     int zlen, nseg, nquads, ntask, nblocks;   // z-segment length and count, x-row quads, waves, blocks
 };
 
-// x-rows per wavefront of the stage kernel (FP64: 4 rows x 16 directions,
-// FP32: 2 rows x 32 directions)
-int rows_per_wave(int esize);
+// x-rows per wavefront of the stage kernel: 4 rows x 16 lanes, each lane
+// loading 8 bytes (FP64: 1 direction, FP32: 2 directions) per field
+constexpr int kRows = 4;
+// directions per chunk = one 128-byte row segment (FP64 16, FP32 32)
+int dirs_per_chunk(int esize);
 
 StagePlan plan_stage(const Geometry& g, int num_cus);
 // the same decomposition with a given z-segment length
@@ -84,7 +89,7 @@ bool buffer_addressable(const Geometry& g, int esize);
 template <typename T>
 hipError_t launch_stage(int stage, const Geometry& g, const Physics& ph, const StagePlan& pl,
                         const T* yin, const T* u, T* w, T* yout, const double* mtot, double* mpart,
-                        double cw, double cy, hipStream_t s);
+                        double cw, double cy, hipStream_t s, double ca = 0.0, double cb = 0.0);
 
 // partial records -> per-site totals (compensated sums for the FP32 form)
 hipError_t launch_moment_totals(const Geometry& g, bool compensated, const double* part, double* tot,

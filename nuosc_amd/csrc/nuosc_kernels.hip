@@ -33,6 +33,10 @@
 
 namespace nuosc {
 
+// directions per lane in the stage kernel: one 8-byte load per field
+template <typename T>
+constexpr int kVals = (int)(sizeof(double) / sizeof(T));
+
 // ---------------------------------------------------------------------------
 // helpers
 // ---------------------------------------------------------------------------
@@ -153,6 +157,28 @@ struct Fields {
     __device__ __forceinline__ void st(unsigned long long b, int f, T x, unsigned vo, unsigned so) const {
         Lane<T>::st(x, rsrc(b, f), vo, so + foff(f));
     }
+    // kVals<T> adjacent values (8 bytes) per lane
+    __device__ __forceinline__ void ldv(unsigned long long b, int f, unsigned vo, unsigned so,
+                                        T (&x)[kVals<T>]) const {
+        const u32x2 r = __builtin_amdgcn_raw_buffer_load_b64(rsrc(b, f), vo, so + foff(f), 0);
+        if constexpr (sizeof(T) == 8) {
+            x[0] = __builtin_bit_cast(double, r);
+        } else {
+            x[0] = __builtin_bit_cast(float, r[0]);
+            x[1] = __builtin_bit_cast(float, r[1]);
+        }
+    }
+    __device__ __forceinline__ void stv(unsigned long long b, int f, const T (&x)[kVals<T>], unsigned vo,
+                                        unsigned so) const {
+        u32x2 r;
+        if constexpr (sizeof(T) == 8) {
+            r = __builtin_bit_cast(u32x2, x[0]);
+        } else {
+            r[0] = __builtin_bit_cast(unsigned, x[0]);
+            r[1] = __builtin_bit_cast(unsigned, x[1]);
+        }
+        __builtin_amdgcn_raw_buffer_store_b64(r, rsrc(b, f), vo, so + foff(f), 0);
+    }
 };
 
 // constant address space: uniform loads from it become scalar loads
@@ -181,11 +207,13 @@ struct StageArgs {
     unsigned long long mpart; // partial moments of yout: [site][nch][kMomRec] (+ one spare x row)
     unsigned mpart_bytes;
     double cw, cy;
+    double ca, cb;            // SSP-RK3 combination a*u + b*(y + dt*k)
     double mu2, mu, p2st, p2ct, pmo, ct2, st;
     int zlen, nquads;
 };
 
-// Sums of 16 values over each group of L consecutive lanes (L = 16, 32, 64):
+// Sums of 16 values (with rounding errors ae carried in the compensated form)
+// over each group of L consecutive lanes (L = 16, 32, 64):
 // four halving exchanges (each lane keeps the half of the values selected by
 // one lane-index bit) and plain butterflies for the remaining bits.  Lane d of
 // a group ends with the group total of value d / (L/16).  Each level has its
@@ -212,7 +240,7 @@ __device__ __forceinline__ void halve_c(const double* in, const double* ine, dou
 }
 
 template <int L, bool COMP>
-__device__ __forceinline__ double group_sum16(const double (&a)[16], int lane) {
+__device__ __forceinline__ double group_sum16(const double (&a)[16], const double (&ae)[16], int lane) {
     constexpr int M1 = L / 2, M2 = L / 4, M3 = L / 8, M4 = L / 16;
     if constexpr (!COMP) {
         double b[8], c[4], d[2], e[1];
@@ -225,10 +253,8 @@ __device__ __forceinline__ double group_sum16(const double (&a)[16], int lane) {
         for (int m = M4 / 2; m >= 1; m >>= 1) x += __shfl_xor(x, m);
         return x;
     } else {
-        double z[16], b[8], be[8], c[4], ce[4], d[2], de[2], e[1], ee[1];
-#pragma unroll
-        for (int k = 0; k < 16; ++k) z[k] = 0.0;
-        halve_c<M1>(a, z, b, be, 8, lane & M1);
+        double b[8], be[8], c[4], ce[4], d[2], de[2], e[1], ee[1];
+        halve_c<M1>(a, ae, b, be, 8, lane & M1);
         halve_c<M2>(b, be, c, ce, 4, lane & M2);
         halve_c<M3>(c, ce, d, de, 2, lane & M3);
         halve_c<M4>(d, de, e, ee, 1, lane & M4);
@@ -244,20 +270,23 @@ __device__ __forceinline__ double group_sum16(const double (&a)[16], int lane) {
     }
 }
 
-// x rows staged per wave: i0-2 .. i0+R+1, L directions each
+// x rows staged per wave: i0-2 .. i0+R+1, D = 16 V directions each
 template <typename T, int R>
 struct Slab {
-    static constexpr int L = kWave / R;
+    static constexpr int D = (kWave / R) * kVals<T>;
     static constexpr int NS = R + 4;
-    T v[2][kNVar][NS * L];   // two z levels (by parity) of all 8 fields
+    T v[2][kNVar][NS * D];   // two z levels (by parity) of all 8 fields
 };
 
 // One wavefront's work.  Block lblock = (direction chunk, group of 4
 // x-row quads, z segment), chunk fastest; wave wib of the block takes quad
-// 4 * group + wib.
+// 4 * group + wib.  Lane (g, dl) of a wave handles x row i0 + g and the V
+// adjacent directions ch * D + dl * V + e, e < V.
 template <typename T, int R, int FPD, int STAGE, bool COMP>
 __device__ __forceinline__ void stage_body(const StageArgs<T, FPD>& a, int lblock, int wib, Slab<T, R>& slab) {
-    constexpr int L = kWave / R;   // directions per row
+    constexpr int V = kVals<T>;    // directions per lane (one 8-byte load)
+    constexpr int L = kWave / R;   // lanes per row
+    constexpr int D = L * V;       // directions per row chunk (one 128-byte line)
     constexpr int NH = 4 / R;      // halo loads per field
     auto& xs = slab.v;
 
@@ -278,11 +307,11 @@ __device__ __forceinline__ void stage_body(const StageArgs<T, FPD>& a, int lbloc
     if (qd >= a.nquads) return;   // last group of a grid with nquads % 4 != 0
     const int i0 = qd * R;
     const int g = lane / L, dl = lane % L;
-    const int v = ch * L + dl;
+    const int p = ch * L + dl;           // direction group: directions p*V .. p*V+V-1
     const int r = i0 + g;
     const bool row_ok = r < nx;
-    const bool act = row_ok && v < nv;
-    const int vl = min(v, nv - 1);
+    const bool act = row_ok && p * V < nv;   // nv is a multiple of V
+    const int pl = min(p, nv / V - 1);       // loads stay inside the directions
     const int rw = wrapi(r, nx);
 
     // lowest referenced row = scalar base; per-lane row deltas (>= 0) in the
@@ -299,41 +328,52 @@ __device__ __forceinline__ void stage_body(const StageArgs<T, FPD>& a, int lbloc
     // stored-row offset of interior z index jj (periodic: wrapped; open: the
     // ghost rows hold jj in [-zoff, nz + zoff))
     auto zrow = [&](int jj) { return (unsigned)(wrapi(jj + zoff, nzs) - zoff); };
-    const unsigned vo = (unsigned)(rw - rmin) * fieldb + (unsigned)vl * sizeof(T);
+    const unsigned vo = (unsigned)(rw - rmin) * fieldb + (unsigned)(pl * V) * sizeof(T);
     // Stores are unconditional (no branch, so the compiler can count the
     // outstanding memory operations across iterations exactly): a lane past
     // the last x row writes the spare row nx of every field, a lane past nv
-    // writes its own padding column; neither is ever read.
+    // writes its own padding columns; neither is ever read.
     const int rst = row_ok ? rw : nx;
-    const unsigned vos = (unsigned)(rst - rmin) * fieldb + (unsigned)v * sizeof(T);
+    const unsigned vos = (unsigned)(rst - rmin) * fieldb + (unsigned)(p * V) * sizeof(T);
     unsigned vh[NH];
     int slot_h[NH];
 #pragma unroll
     for (int h = 0; h < NH; ++h) {
         const int k = h * R + g;   // halo index 0..3 -> rows i0-2, i0-1, i0+R, i0+R+1
         const int hr = wrapi(k < 2 ? i0 - 2 + k : i0 + R + (k - 2), nx);
-        vh[h] = (unsigned)(hr - rmin) * fieldb + (unsigned)vl * sizeof(T);
-        slot_h[h] = (k < 2 ? k : R + k) * L + dl;
+        vh[h] = (unsigned)(hr - rmin) * fieldb + (unsigned)(pl * V) * sizeof(T);
+        slot_h[h] = (k < 2 ? k : R + k) * D + dl * V;
     }
-    const int slot_c = (g + 2) * L + dl;
+    const int slot_c = (g + 2) * D + dl * V;
 
     // per-direction constants (nuosc_class.cpp:119-121, 180-181)
-    const double vxd = a.g.vx[vl], vyd = a.g.vy[vl], vzd = a.g.vz[vl];
-    const double vwd = act ? a.g.vw[vl] : 0.0;
-    const T fx = (T)(-vxd / a.g.twelve_dx);
-    const T fz = (T)(-vzd / a.g.twelve_dz);
+    double vxd[V], vyd[V], vzd[V], q[V][4];
+    T fx[V], fz[V];
+#pragma unroll
+    for (int e = 0; e < V; ++e) {
+        const int ve = pl * V + e;
+        vxd[e] = a.g.vx[ve];
+        vyd[e] = a.g.vy[ve];
+        vzd[e] = a.g.vz[ve];
+        const double vwd = act ? a.g.vw[ve] : 0.0;
+        fx[e] = (T)(-vxd[e] / a.g.twelve_dx);
+        fz[e] = (T)(-vzd[e] / a.g.twelve_dz);
+        q[e][0] = vwd;
+        q[e][1] = vxd[e] * vwd;
+        q[e][2] = vyd[e] * vwd;
+        q[e][3] = vzd[e] * vwd;
+    }
     const T kx = (T)a.g.ko_x, kz = (T)a.g.ko_z;
-    const double q[4] = {vwd, vxd * vwd, vyd * vwd, vzd * vwd};
     const T mu2 = (T)a.mu2, mu = (T)a.mu, p2st = (T)a.p2st, p2ct = (T)a.p2ct;
     const T pmo = (T)a.pmo, ct2 = (T)a.ct2, st = (T)a.st;
-    const T cw = (T)a.cw, cy = (T)a.cy;
+    const T cw = (T)a.cw, cy = (T)a.cy, ca = (T)a.ca, cb = (T)a.cb;
     const unsigned vom = (unsigned)(rw - rmin) * (unsigned)nz * (kMomStride * 8);
 
     const int j0 = zs * a.zlen;
     const int j1 = min(nz, j0 + a.zlen);
 
-    T win[kNVar][5];   // own row, z = j-2 .. j+2
-    T hreg[NH][kNVar]; // halo rows of z = j+2, staged into LDS at the end of step j
+    T win[kNVar][5][V];   // own row, z = j-2 .. j+2
+    T hreg[NH][kNVar][V]; // halo rows of z = j+2, staged into LDS at the end of step j
     // velocity moments of the stage input at this lane's site, z = j (M) and
     // z = j+1 (Mn, in flight during step j)
     double M[kNMom], Mn[kNMom];
@@ -353,7 +393,7 @@ __device__ __forceinline__ void stage_body(const StageArgs<T, FPD>& a, int lbloc
     for (int qz = 0; qz < 4; ++qz) {
         const unsigned so = (rbase + zrow(j0 - 2 + qz)) * rowb;
 #pragma unroll
-        for (int f = 0; f < kNVar; ++f) win[f][qz] = a.yin.ld(a.yin.base, f, vo, so);
+        for (int f = 0; f < kNVar; ++f) a.yin.ldv(a.yin.base, f, vo, so, win[f][qz]);
     }
     // halo rows of z = j0 and j0+1 straight into their levels
 #pragma unroll
@@ -362,8 +402,12 @@ __device__ __forceinline__ void stage_body(const StageArgs<T, FPD>& a, int lbloc
 #pragma unroll
         for (int h = 0; h < NH; ++h)
 #pragma unroll
-            for (int f = 0; f < kNVar; ++f)
-                xs[(j0 + qz) & 1][f][slot_h[h]] = a.yin.ld(a.yin.base, f, vh[h], so);
+            for (int f = 0; f < kNVar; ++f) {
+                T x[V];
+                a.yin.ldv(a.yin.base, f, vh[h], so, x);
+#pragma unroll
+                for (int e = 0; e < V; ++e) xs[(j0 + qz) & 1][f][slot_h[h] + e] = x[e];
+            }
     }
     load_moments(M, j0);
 
@@ -373,78 +417,86 @@ __device__ __forceinline__ void stage_body(const StageArgs<T, FPD>& a, int lbloc
         wave_sync();   // the previous iteration's LDS reads stay before these writes
         // own rows of z = j into this level (the halo rows of z = j are there)
 #pragma unroll
-        for (int f = 0; f < kNVar; ++f) xs[lv][f][slot_c] = win[f][2];
+        for (int f = 0; f < kNVar; ++f)
+#pragma unroll
+            for (int e = 0; e < V; ++e) xs[lv][f][slot_c + e] = win[f][2][e];
         // ---- all loads of the iteration, in the order they are consumed:
         // own row z+2 (stencil), u and w (update), halo rows z+2 (next
         // iteration), moments of z+1 (next iteration)
         const unsigned long long by = a.yin.fresh_base();
         const unsigned s2 = (rbase + zrow(j + 2)) * rowb;
 #pragma unroll
-        for (int f = 0; f < kNVar; ++f) win[f][4] = a.yin.ld(by, f, vo, s2);
-        T uc[kNVar], wc[kNVar];
-        if constexpr (STAGE == 2 || STAGE == 3) {
+        for (int f = 0; f < kNVar; ++f) a.yin.ldv(by, f, vo, s2, win[f][4]);
+        T uc[kNVar][V], wc[kNVar][V];
+        if constexpr (STAGE == 2 || STAGE == 3 || STAGE == 6 || STAGE == 7) {
             const unsigned long long bu = a.u.fresh_base();
 #pragma unroll
-            for (int f = 0; f < kNVar; ++f) uc[f] = a.u.ld(bu, f, vo, so);
+            for (int f = 0; f < kNVar; ++f) a.u.ldv(bu, f, vo, so, uc[f]);
         }
-        if constexpr (STAGE >= 2) {
+        if constexpr (STAGE >= 2 && STAGE <= 4) {
             const unsigned long long bw = a.w.fresh_base();
 #pragma unroll
-            for (int f = 0; f < kNVar; ++f) wc[f] = a.w.ld(bw, f, vo, so);
+            for (int f = 0; f < kNVar; ++f) a.w.ldv(bw, f, vo, so, wc[f]);
         }
 #pragma unroll
         for (int h = 0; h < NH; ++h)
 #pragma unroll
-            for (int f = 0; f < kNVar; ++f) hreg[h][f] = a.yin.ld(by, f, vh[h], s2);
+            for (int f = 0; f < kNVar; ++f) a.yin.ldv(by, f, vh[h], s2, hreg[h][f]);
         // unconditional (a branch here would make every later wait a full
         // drain); the last row of the box reloads itself
         load_moments(Mn, min(j + 1, nz - 1));
         wave_sync();   // other lanes' writes are visible to the reads below
-        // projections onto direction v (INP, nuosc_class.cpp:187)
-        const T A = (T)(M[0] - M[1] * vxd - M[2] * vyd - M[3] * vzd);
-        const T B = (T)(M[4] - M[5] * vxd - M[6] * vyd - M[7] * vzd);
-        const T C = (T)(M[8] - M[9] * vxd - M[10] * vyd - M[11] * vzd);
 
-        const T ee = win[0][2], mm = win[1][2], emr = win[2][2], emi = win[3][2];
-        const T bee = win[4][2], bmm = win[5][2], bemr = win[6][2], bemi = win[7][2];
-        // interaction (nuosc_class.cpp:193-200)
-        const T Iee = mu2 * (emi * A + emr * B);
-        const T Iemr = -mu * ((ee - mm) * B + C * emi);
-        const T Iemi = mu * ((mm - ee) * A + C * emr);
-        const T Ibee = mu2 * (bemi * A + bemr * B);
-        const T Ibemr = -mu * ((bee - bmm) * B + C * bemi);
-        const T Ibemi = mu * ((bmm - bee) * A + C * bemr);
-        // local terms (interaction + vacuum, nuosc_class.cpp:280-287)
-        const T loc[kNVar] = {Iee, -Iee, Iemr, Iemi, Ibee, -Ibee, Ibemr, Ibemi};
-        const T vac[kNVar] = {-(p2st * emi), p2st * emi, -(p2ct * emi), pmo * (ct2 * emr + st * (ee - mm)),
-                              -(p2st * bemi), p2st * bemi, -(p2ct * bemi), pmo * (ct2 * bemr + st * (bee - bmm))};
-        T k[kNVar];
+        T yo[kNVar][V], wn[kNVar][V];
 #pragma unroll
-        for (int f = 0; f < kNVar; ++f) {
-            const T xm2 = xs[lv][f][slot_c - 2 * L], xm1 = xs[lv][f][slot_c - L];
-            const T xp1 = xs[lv][f][slot_c + L], xp2 = xs[lv][f][slot_c + 2 * L];
-            const T zm2 = win[f][0], zm1 = win[f][1], c0 = win[f][2], zp1 = win[f][3], zp2 = win[f][4];
-            // advection (nuosc_class.cpp:182-183)
-            const T adv = fz * ((zm2 - zp2) - (T)8.0 * (zm1 - zp1)) + fx * ((xm2 - xp2) - (T)8.0 * (xm1 - xp1));
-            // Kreiss-Oliger (nuosc_class.cpp:122-123)
-            const T ko = kz * (zm2 + zp2 - (T)4 * (zm1 + zp1) + (T)6 * c0) +
-                         kx * (xm2 + xp2 - (T)4 * (xm1 + xp1) + (T)6 * c0);
-            k[f] = loc[f] + adv + ko + vac[f];
-        }
+        for (int e = 0; e < V; ++e) {
+            // projections onto direction v (INP, nuosc_class.cpp:187)
+            const T A = (T)(M[0] - M[1] * vxd[e] - M[2] * vyd[e] - M[3] * vzd[e]);
+            const T B = (T)(M[4] - M[5] * vxd[e] - M[6] * vyd[e] - M[7] * vzd[e]);
+            const T C = (T)(M[8] - M[9] * vxd[e] - M[10] * vyd[e] - M[11] * vzd[e]);
 
-        T yo[kNVar], wn[kNVar];
+            const T ee = win[0][2][e], mm = win[1][2][e], emr = win[2][2][e], emi = win[3][2][e];
+            const T bee = win[4][2][e], bmm = win[5][2][e], bemr = win[6][2][e], bemi = win[7][2][e];
+            // interaction (nuosc_class.cpp:193-200)
+            const T Iee = mu2 * (emi * A + emr * B);
+            const T Iemr = -mu * ((ee - mm) * B + C * emi);
+            const T Iemi = mu * ((mm - ee) * A + C * emr);
+            const T Ibee = mu2 * (bemi * A + bemr * B);
+            const T Ibemr = -mu * ((bee - bmm) * B + C * bemi);
+            const T Ibemi = mu * ((bmm - bee) * A + C * bemr);
+            // local terms (interaction + vacuum, nuosc_class.cpp:280-287)
+            const T loc[kNVar] = {Iee, -Iee, Iemr, Iemi, Ibee, -Ibee, Ibemr, Ibemi};
+            const T vac[kNVar] = {-(p2st * emi), p2st * emi, -(p2ct * emi), pmo * (ct2 * emr + st * (ee - mm)),
+                                  -(p2st * bemi), p2st * bemi, -(p2ct * bemi),
+                                  pmo * (ct2 * bemr + st * (bee - bmm))};
 #pragma unroll
-        for (int f = 0; f < kNVar; ++f) {
-            if constexpr (STAGE == 0) {
-                yo[f] = k[f];
-            } else if constexpr (STAGE == 1) {
-                wn[f] = win[f][2] + cw * k[f];
-                yo[f] = win[f][2] + cy * k[f];
-            } else if constexpr (STAGE == 2 || STAGE == 3) {
-                wn[f] = wc[f] + cw * k[f];
-                yo[f] = uc[f] + cy * k[f];
-            } else {
-                yo[f] = wc[f] + cw * k[f];
+            for (int f = 0; f < kNVar; ++f) {
+                const T xm2 = xs[lv][f][slot_c - 2 * D + e], xm1 = xs[lv][f][slot_c - D + e];
+                const T xp1 = xs[lv][f][slot_c + D + e], xp2 = xs[lv][f][slot_c + 2 * D + e];
+                const T zm2 = win[f][0][e], zm1 = win[f][1][e], c0 = win[f][2][e], zp1 = win[f][3][e],
+                        zp2 = win[f][4][e];
+                // advection (nuosc_class.cpp:182-183)
+                const T adv = fz[e] * ((zm2 - zp2) - (T)8.0 * (zm1 - zp1)) +
+                              fx[e] * ((xm2 - xp2) - (T)8.0 * (xm1 - xp1));
+                // Kreiss-Oliger (nuosc_class.cpp:122-123)
+                const T ko = kz * (zm2 + zp2 - (T)4 * (zm1 + zp1) + (T)6 * c0) +
+                             kx * (xm2 + xp2 - (T)4 * (xm1 + xp1) + (T)6 * c0);
+                const T k = loc[f] + adv + ko + vac[f];
+                if constexpr (STAGE == 0) {
+                    yo[f][e] = k;
+                } else if constexpr (STAGE == 1) {
+                    wn[f][e] = c0 + cw * k;
+                    yo[f][e] = c0 + cy * k;
+                } else if constexpr (STAGE == 2 || STAGE == 3) {
+                    wn[f][e] = wc[f][e] + cw * k;
+                    yo[f][e] = uc[f][e] + cy * k;
+                } else if constexpr (STAGE == 4) {
+                    yo[f][e] = wc[f][e] + cw * k;
+                } else if constexpr (STAGE == 5) {   // SSP-RK3 (nuosc_class.cpp:436)
+                    yo[f][e] = c0 + cy * k;
+                } else {                             // SSP-RK3 (nuosc_class.cpp:440, 444, 369)
+                    yo[f][e] = ca * uc[f][e] + cb * (c0 + cy * k);
+                }
             }
         }
         {
@@ -452,35 +504,63 @@ __device__ __forceinline__ void stage_body(const StageArgs<T, FPD>& a, int lbloc
             const unsigned long long bo = a.yout.fresh_base();
 #pragma unroll
             for (int f = 0; f < kNVar; ++f) {
-                if constexpr (STAGE >= 1 && STAGE <= 3) a.w.st(bw, f, wn[f], vos, so);
-                a.yout.st(bo, f, yo[f], vos, so);
+                if constexpr (STAGE >= 1 && STAGE <= 3) a.w.stv(bw, f, wn[f], vos, so);
+                a.yout.stv(bo, f, yo[f], vos, so);
             }
         }
         if constexpr (STAGE != 0) {
-            double t[16];
-            moment_terms<T>(yo, q, t);
-            const double tot = group_sum16<L, COMP>(t, lane);
-            // lane dl of a row holds value dl >> sh; every lane stores it
-            // (lanes sharing a value store the same bits; values 12..15 fill
-            // unused slots of the 16-double record)
-            constexpr int sh = (L == 16) ? 0 : (L == 32 ? 1 : 2);
+            // moment terms of the lane's V directions, summed per lane first
+            // (exactly, with the rounding error carried, in the FP32 form)
+            double t[16], te[16];
+            {
+                T y0[kNVar];
+#pragma unroll
+                for (int f = 0; f < kNVar; ++f) y0[f] = yo[f][0];
+                moment_terms<T>(y0, q[0], t);
+            }
+#pragma unroll
+            for (int s = 0; s < 16; ++s) te[s] = 0.0;
+#pragma unroll
+            for (int e = 1; e < V; ++e) {
+                T ye[kNVar];
+#pragma unroll
+                for (int f = 0; f < kNVar; ++f) ye[f] = yo[f][e];
+                double u[16];
+                moment_terms<T>(ye, q[e], u);
+#pragma unroll
+                for (int s = 0; s < 16; ++s) {
+                    if constexpr (COMP) {
+                        double s2, err;
+                        two_sum(t[s], u[s], s2, err);
+                        t[s] = s2;
+                        te[s] += err;
+                    } else {
+                        t[s] += u[s];
+                    }
+                }
+            }
+            // lane dl of a row holds the row total of value dl (16 lanes per
+            // row, 12 values used; values 12..15 fill unused record slots)
+            const double tot = group_sum16<L, COMP>(t, te, lane);
             unsigned long long bp = a.mpart;
             asm volatile("" : "+s"(bp));
             const __amdgpu_buffer_rsrc_t rp =
                 __builtin_amdgcn_make_buffer_rsrc((void*)bp, 0, (int)a.mpart_bytes, kRsrcWord3);
             const unsigned rec = ((unsigned)(rst - rmin) * (unsigned)nz * nch + (unsigned)ch) * (kMomRec * 8);
             const unsigned srec = (mbase + j) * (unsigned)(nch * kMomRec * 8);
-            Lane<double>::st(tot, rp, rec + (unsigned)(dl >> sh) * 8, srec);
+            Lane<double>::st(tot, rp, rec + (unsigned)dl * 8, srec);
         }
 #pragma unroll
-        for (int f = 0; f < kNVar; ++f) {
-            win[f][0] = win[f][1];
-            win[f][1] = win[f][2];
-            win[f][2] = win[f][3];
-            win[f][3] = win[f][4];
-        }
+        for (int f = 0; f < kNVar; ++f)
 #pragma unroll
-        for (int q = 0; q < kNMom; ++q) M[q] = Mn[q];
+            for (int e = 0; e < V; ++e) {
+                win[f][0][e] = win[f][1][e];
+                win[f][1][e] = win[f][2][e];
+                win[f][2][e] = win[f][3][e];
+                win[f][3][e] = win[f][4][e];
+            }
+#pragma unroll
+        for (int s = 0; s < kNMom; ++s) M[s] = Mn[s];
         // halo rows of z = j+2 (loaded in this iteration) into this level,
         // whose z = j reads are done; level lv ^ 1 holds z = j+1.  Done here
         // rather than at the top of the next iteration: a wait at the loop
@@ -490,7 +570,9 @@ __device__ __forceinline__ void stage_body(const StageArgs<T, FPD>& a, int lbloc
 #pragma unroll
         for (int h = 0; h < NH; ++h)
 #pragma unroll
-            for (int f = 0; f < kNVar; ++f) xs[lv][f][slot_h[h]] = hreg[h][f];
+            for (int f = 0; f < kNVar; ++f)
+#pragma unroll
+                for (int e = 0; e < V; ++e) xs[lv][f][slot_h[h] + e] = hreg[h][f][e];
     }
 }
 
@@ -515,13 +597,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
     stage_body<T, R, FPD, STAGE, COMP>(a, blk, wib, slab[wib]);
 }
 
-int rows_per_wave(int esize) { return esize == 8 ? 4 : 2; }
+int dirs_per_chunk(int esize) { return (kWave / kRows) * (8 / esize); }
 
 StagePlan plan_stage(const Geometry& g, int num_cus) {
     // enough waves to give every SIMD several (>= 12 per CU), segments long
     // enough that the window warm-up stays a small overhead
     StagePlan p{};
-    const int R = kWave / g.lpr;
+    const int R = kRows;
     const long long quads = (g.nx + R - 1) / R;
     const long long rows = quads * g.nch;
     const long long want = (long long)num_cus * 48;
@@ -538,7 +620,7 @@ StagePlan plan_stage(const Geometry& g, int num_cus) {
 
 StagePlan plan_stage_zlen(const Geometry& g, int zlen) {
     StagePlan p{};
-    const int R = kWave / g.lpr;
+    const int R = kRows;
     const long long quads = (g.nx + R - 1) / R;
     p.zlen = std::max(1, std::min(zlen, g.nz));
     p.nseg = (g.nz + p.zlen - 1) / p.zlen;
@@ -573,7 +655,7 @@ static Fields<T, FPD> fields_of(const Geometry& g, const T* p) {
 template <typename T, int FPD>
 static StageArgs<T, FPD> make_stage_args(const Geometry& g, const Physics& ph, const StagePlan& pl, const T* yin,
                                          const T* u, T* w, T* yout, const double* mtot, double* mpart, double cw,
-                                         double cy) {
+                                         double cy, double ca = 0.0, double cb = 0.0) {
     StageArgs<T, FPD> a;
     a.g = g;
     a.yin = fields_of<T, FPD>(g, yin);
@@ -586,6 +668,8 @@ static StageArgs<T, FPD> make_stage_args(const Geometry& g, const Physics& ph, c
     a.mpart_bytes = (unsigned)((long long)(g.nx + 1) * g.nz * g.nch * kMomRec * sizeof(double));
     a.cw = cw;
     a.cy = cy;
+    a.ca = ca;
+    a.cb = cb;
     a.mu2 = 2 * ph.mu;
     a.mu = ph.mu;
     a.p2st = ph.pmo * 2 * ph.st;
@@ -601,11 +685,11 @@ static StageArgs<T, FPD> make_stage_args(const Geometry& g, const Physics& ph, c
 template <typename T, int FPD>
 static hipError_t launch_stage_fpd(int stage, const Geometry& g, const Physics& ph, const StagePlan& pl,
                                    const T* yin, const T* u, T* w, T* yout, const double* mtot, double* mpart,
-                                   double cw, double cy, hipStream_t s) {
-    const StageArgs<T, FPD> a = make_stage_args<T, FPD>(g, ph, pl, yin, u, w, yout, mtot, mpart, cw, cy);
+                                   double cw, double cy, hipStream_t s, double ca, double cb) {
+    const StageArgs<T, FPD> a = make_stage_args<T, FPD>(g, ph, pl, yin, u, w, yout, mtot, mpart, cw, cy, ca, cb);
     constexpr bool COMP = sizeof(T) == 4;
-    constexpr int R = sizeof(T) == 8 ? 4 : 2;
-    if (g.lpr != kWave / R) return hipErrorInvalidValue;
+    constexpr int R = kRows;
+    if (g.lpr != dirs_per_chunk((int)sizeof(T))) return hipErrorInvalidValue;
     dim3 grid(pl.nblocks), block(256);
     switch (stage) {
         case 0: hipLaunchKernelGGL((k_stage<T, R, FPD, 0, COMP>), grid, block, 0, s, a); break;
@@ -613,6 +697,9 @@ static hipError_t launch_stage_fpd(int stage, const Geometry& g, const Physics& 
         case 2: hipLaunchKernelGGL((k_stage<T, R, FPD, 2, COMP>), grid, block, 0, s, a); break;
         case 3: hipLaunchKernelGGL((k_stage<T, R, FPD, 3, COMP>), grid, block, 0, s, a); break;
         case 4: hipLaunchKernelGGL((k_stage<T, R, FPD, 4, COMP>), grid, block, 0, s, a); break;
+        case 5: hipLaunchKernelGGL((k_stage<T, R, FPD, 5, COMP>), grid, block, 0, s, a); break;
+        case 6: hipLaunchKernelGGL((k_stage<T, R, FPD, 6, COMP>), grid, block, 0, s, a); break;
+        case 7: hipLaunchKernelGGL((k_stage<T, R, FPD, 7, COMP>), grid, block, 0, s, a); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -621,12 +708,12 @@ static hipError_t launch_stage_fpd(int stage, const Geometry& g, const Physics& 
 template <typename T>
 hipError_t launch_stage(int stage, const Geometry& g, const Physics& ph, const StagePlan& pl,
                         const T* yin, const T* u, T* w, T* yout, const double* mtot, double* mpart,
-                        double cw, double cy, hipStream_t s) {
+                        double cw, double cy, hipStream_t s, double ca, double cb) {
     if (!buffer_addressable(g, sizeof(T))) return hipErrorInvalidValue;
     switch (fields_per_descriptor(g, sizeof(T))) {
-        case 4: return launch_stage_fpd<T, 4>(stage, g, ph, pl, yin, u, w, yout, mtot, mpart, cw, cy, s);
-        case 2: return launch_stage_fpd<T, 2>(stage, g, ph, pl, yin, u, w, yout, mtot, mpart, cw, cy, s);
-        default: return launch_stage_fpd<T, 1>(stage, g, ph, pl, yin, u, w, yout, mtot, mpart, cw, cy, s);
+        case 4: return launch_stage_fpd<T, 4>(stage, g, ph, pl, yin, u, w, yout, mtot, mpart, cw, cy, s, ca, cb);
+        case 2: return launch_stage_fpd<T, 2>(stage, g, ph, pl, yin, u, w, yout, mtot, mpart, cw, cy, s, ca, cb);
+        default: return launch_stage_fpd<T, 1>(stage, g, ph, pl, yin, u, w, yout, mtot, mpart, cw, cy, s, ca, cb);
     }
 }
 
@@ -761,7 +848,7 @@ hipError_t batch_build(BatchTables& t, const std::vector<BatchMember>& ms, int e
     for (const auto& m : ms) {
         if (!buffer_addressable(m.g, esize)) return hipErrorInvalidValue;
         fpd = std::min(fpd, fields_per_descriptor(m.g, esize));
-        if (m.g.lpr != kWave / rows_per_wave(esize)) return hipErrorInvalidValue;
+        if (m.g.lpr != dirs_per_chunk(esize)) return hipErrorInvalidValue;
     }
     t.fpd = fpd;
     if (esize == 8) {
@@ -788,7 +875,7 @@ void batch_free(BatchTables& t) {
 
 template <typename T, int FPD>
 static hipError_t batch_step_t(const BatchTables& t, hipStream_t s) {
-    constexpr int R = sizeof(T) == 8 ? 4 : 2;
+    constexpr int R = kRows;
     constexpr bool COMP = sizeof(T) == 4;
     const dim3 grid(t.nblocks), block(256), tgrid(t.tblocks);
     for (int st = 1; st <= 4; ++st) {
@@ -806,7 +893,7 @@ static hipError_t batch_step_t(const BatchTables& t, hipStream_t s) {
 
 template <typename T, int FPD>
 static hipError_t batch_stage_t(const BatchTables& t, int st, hipStream_t s) {
-    constexpr int R = sizeof(T) == 8 ? 4 : 2;
+    constexpr int R = kRows;
     constexpr bool COMP = sizeof(T) == 4;
     const dim3 grid(t.nblocks), block(256);
     const StageArgs<T, FPD>* a = (const StageArgs<T, FPD>*)t.args[st - 1];
@@ -1292,7 +1379,7 @@ hipError_t launch_advance_time(double* t_dev, double dt, hipStream_t s) {
 #define NUOSC_INST(T)                                                                                   \
     template hipError_t launch_stage<T>(int, const Geometry&, const Physics&, const StagePlan&, const T*, \
                                         const T*, T*, T*, const double*, double*, double, double,        \
-                                        hipStream_t);                                                    \
+                                        hipStream_t, double, double);                                    \
     template hipError_t launch_moments<T>(const Geometry&, const T*, double*, hipStream_t);              \
     template hipError_t launch_init<T>(const Geometry&, int, const double*, const double*, const double*, \
                                        const double*, double, uint64_t, T*, hipStream_t);                \

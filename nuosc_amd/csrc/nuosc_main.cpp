@@ -3,8 +3,9 @@
 // analysis.dat, wall-time lines, [Summ] line).  Additions: --precision 64|32,
 // --seed S (for --ipt 5), --vw voronoi|equal, --snap_p3 1 (P3 .bin/.meta
 // snapshot every DUMP_EVERY steps), --graph 0|1, --device N, --bc
-// periodic|open (open: z ends open, an ipt-0 pulse flowing in) and --zc Z
-// (centre of the ipt-0 profile, 0 in the reference).
+// periodic|open (open: z ends open, an ipt-0 pulse flowing in), --zc Z
+// (centre of the ipt-0 profile, 0 in the reference) and --srk3 1 (step with
+// NuOsc::step_srk3 instead of step_rk4).
 #include <sys/resource.h>
 
 #include <chrono>
@@ -42,7 +43,7 @@ int main(int argc, char* argv[]) {
     int gz = 2, gx = 2;
     // additions
     int precision = NUOSC_PREC_FP64, vgrid = NUOSC_VGRID_VORONOI, snap_p3 = 0, use_graph = 1, device = 0;
-    int bc = NUOSC_BC_PERIODIC;
+    int bc = NUOSC_BC_PERIODIC, srk3 = 0;
     double zc = 0.0;
     unsigned long long seed = 12345;
 
@@ -102,6 +103,7 @@ int main(int argc, char* argv[]) {
             else { std::printf("Unreconganized parameters %s!\n", m); std::exit(0); }
         }
         else if (!std::strcmp(a, "--zc")) zc = std::atof(val());
+        else if (!std::strcmp(a, "--srk3")) srk3 = std::atoi(val());
         else {
             // nuosc.cpp:135-137 (message and exit status kept)
             std::printf("Unreconganized parameters %s!\n", a);
@@ -141,7 +143,8 @@ int main(int argc, char* argv[]) {
                 state.synchronize();
                 t1 = std::chrono::high_resolution_clock::now();
             }
-            state.step_rk4();
+            if (srk3) state.step_srk3();
+            else state.step_rk4();
             if (t % ANAL_EVERY == 0) state.analysis();
             if (t % DUMP_EVERY == 0) state.checkSnapShot(t);
             if (t == 5 || t == 10 || t == 100 || t == 1000 || t == END_STEP) {

@@ -530,6 +530,16 @@ static void vec2(orc_sim* s, double** v0, double** v1, double a, double** v2, do
                 for (int f = 0; f < ORC_NVAR; ++f) v0[f][k] = v1[f][k] + a * (v2[f][k] + v3[f][k]);
             }
 }
+/* v0 = a*v1 + b*(v2 + dt*v3) over the interior (nuosc_class.cpp:353-372) */
+static void vec3(orc_sim* s, double** v0, double a, double** v1, double b, double** v2, double dt, double** v3) {
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int i = 0; i < s->nx; ++i)
+        for (int j = 0; j < s->nz; ++j)
+            for (int v = 0; v < s->nv; ++v) {
+                long k = orc_idx(s, i, j, v);
+                for (int f = 0; f < ORC_NVAR; ++f) v0[f][k] = a * v1[f][k] + b * (v2[f][k] + dt * v3[f][k]);
+            }
+}
 static void swapbuf(double** a, double** b) {
     for (int f = 0; f < ORC_NVAR; ++f) { double* t = a[f]; a[f] = b[f]; b[f] = t; }
 }
@@ -557,6 +567,25 @@ void orc_step_rk4(orc_sim* s) {
     orc_calrhs(s, s->cor, s->pre);
     vec2(s, s->pre, s->stat, 1.0 / 6.0 * dt, s->cor, s->rhs);
     swapbuf(s->pre, s->stat);
+    if (s->renorm) orc_renormalize(s, s->stat);
+    s->phy_time += dt;
+}
+
+/* SSP-RK3 (nuosc_class.cpp:429-452): same stage combinations and buffer
+ * use.  The reference's step_srk3 refreshes no ghost cells before its RHS
+ * evaluations; here each stage input gets the boundary fill of step_rk4 (at
+ * the stage times t, t+dt, t+dt/2). */
+void orc_step_srk3(orc_sim* s) {
+    const double dt = s->dt, t = s->phy_time;
+    orc_fill_ghosts(s, s->stat, t);
+    orc_calrhs(s, s->rhs, s->stat);
+    vec1(s, s->pre, s->stat, dt, s->rhs);                              /* u1 = u0 + dt L(u0) */
+    orc_fill_ghosts(s, s->pre, t + dt);
+    orc_calrhs(s, s->rhs, s->pre);
+    vec3(s, s->pre, 3.0 / 4.0, s->stat, 1.0 / 4.0, s->pre, dt, s->rhs); /* u2 */
+    orc_fill_ghosts(s, s->pre, t + 0.5 * dt);
+    orc_calrhs(s, s->cor, s->pre);
+    vec3(s, s->stat, 1.0 / 3.0, s->stat, 2.0 / 3.0, s->pre, dt, s->cor);
     if (s->renorm) orc_renormalize(s, s->stat);
     s->phy_time += dt;
 }

@@ -22,6 +22,8 @@
  *   orc_calrhs              nuosc_class.cpp:11-307 (2 flavours, KO_ORD_3,
  *                           central 4th-order advection, vacuum on)
  *   orc_step_rk4            nuosc_class.cpp:312-351, 374-427
+ *   orc_step_srk3           nuosc_class.cpp:353-372, 429-452 (with boundary
+ *                           fills, which the reference's step_srk3 omits)
  *   orc_analysis            nuosc_ana.cpp:3-137
  *   orc_renormalize         nuosc_ana.cpp:140-166
  *
@@ -34,6 +36,8 @@
  *     H = mu*sum_k w_k(1-v.v_k)(rho_k - conj(rhobar_k)) + pmo(st s1 - ct s3),
  *     rhs = -i[H,rho], np.roll finite differences); single-Fourier-mode symbol
  *     of ADV+KO.
+ *   - SSP-RK3: exact amplification (1 + z + z^2/2 + z^3/6)^n of a Fourier
+ *     mode.
  *   - RK4: exact amplification G(z)^n of a Fourier mode; vacuum precession
  *     polynomial; conservation of sum_xz(ee+mm) per direction.
  *   - init: sum_v w G0 = 1, sum_v w G0b = alpha, trace and purity identities.
@@ -121,6 +125,8 @@ void orc_open_ghosts(orc_sim* s, double* const f[ORC_NVAR], double t);
 void orc_fill_ghosts(orc_sim* s, double* const f[ORC_NVAR], double t);
 void orc_calrhs(const orc_sim* s, double* const out[ORC_NVAR], double* const in[ORC_NVAR]);
 void orc_step_rk4(orc_sim* s);
+/* SSP-RK3 (nuosc_class.cpp:429-452) with a boundary fill before each RHS. */
+void orc_step_srk3(orc_sim* s);
 void orc_renormalize(orc_sim* s, double* const f[ORC_NVAR]);
 
 /* Diagnostics (nuosc_ana.cpp:34-137).  out[9] = {phy_time, maxdP, surv, survb,

@@ -73,6 +73,7 @@ def lib():
         L.orc_fill_ghosts.argtypes = [C.POINTER(_Sim), dp * 8, C.c_double]
         L.orc_calrhs.argtypes = [C.POINTER(_Sim), dp * 8, dp * 8]
         L.orc_step_rk4.argtypes = [C.POINTER(_Sim)]
+        L.orc_step_srk3.argtypes = [C.POINTER(_Sim)]
         L.orc_renormalize.argtypes = [C.POINTER(_Sim), dp * 8]
         L.orc_analysis.argtypes = [C.POINTER(_Sim), dp]
         L.orc_analysis.restype = C.c_int
@@ -185,6 +186,10 @@ class OracleSim:
     def step(self, n=1):
         for _ in range(n):
             lib().orc_step_rk4(self._p)
+
+    def step_srk3(self, n=1):
+        for _ in range(n):
+            lib().orc_step_srk3(self._p)
 
     def analysis(self):
         out = np.zeros(9)
