@@ -45,6 +45,7 @@ def parse():
     p.add_argument("--zmax", type=float, default=12.8)
     p.add_argument("--dz", type=float, default=0.05)
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--no-autotune", action="store_true", help="box: keep the default z-segment plan")
     p.add_argument("--cpu-seconds", type=float, default=12.0)
     p.add_argument("--workload", default="box", choices=("box", "batched", "served"),
                    help="box: one large run (default); batched: 64 uneven runs stepped together vs one by one; "
@@ -269,6 +270,7 @@ def main():
     g.fillInitValue(ipt=5, alpha=0.9, eps0=1e-6, sigma=1.0, lnue=0.6, lnueb=0.5, seed=12345)
     nx, nz, nv = g.shape
     upd = nx * nz * nv
+    zlen = g.plan()[0] if a.no_autotune else g.autotune(trials=2)[0]
 
     g.step_rk4(a.warmup)
     g.synchronize()
@@ -305,7 +307,7 @@ def main():
         "config": {
             "workload": f"{nx}x{nz} periodic box x {nv} icosahedral directions, "
                         f"{'FP64' if a.precision == 64 else 'FP32 + FP64 moments'}, mu=1 ko=0.1 cfl=0.4 pmo=0",
-            "nx": nx, "nz": nz, "nv": nv, "dz": a.dz,
+            "nx": nx, "nz": nz, "nv": nv, "dz": a.dz, "zlen": zlen,
         },
         "roofline": {
             "bound": "hbm",

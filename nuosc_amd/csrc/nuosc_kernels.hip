@@ -157,15 +157,19 @@ struct Fields {
     __device__ __forceinline__ void st(unsigned long long b, int f, T x, unsigned vo, unsigned so) const {
         Lane<T>::st(x, rsrc(b, f), vo, so + foff(f));
     }
-    // kVals<T> adjacent values (8 bytes) per lane
+    // kVals<T> adjacent values (8 bytes) per lane.  The two floats are split
+    // off through a 64-bit integer: taking the elements of the loaded vector
+    // directly makes this compiler narrow the load to its first dword and
+    // return that dword for both.
     __device__ __forceinline__ void ldv(unsigned long long b, int f, unsigned vo, unsigned so,
                                         T (&x)[kVals<T>]) const {
         const u32x2 r = __builtin_amdgcn_raw_buffer_load_b64(rsrc(b, f), vo, so + foff(f), 0);
         if constexpr (sizeof(T) == 8) {
             x[0] = __builtin_bit_cast(double, r);
         } else {
-            x[0] = __builtin_bit_cast(float, r[0]);
-            x[1] = __builtin_bit_cast(float, r[1]);
+            const unsigned long long w = __builtin_bit_cast(unsigned long long, r);
+            x[0] = __builtin_bit_cast(float, (unsigned)w);
+            x[1] = __builtin_bit_cast(float, (unsigned)(w >> 32));
         }
     }
     __device__ __forceinline__ void stv(unsigned long long b, int f, const T (&x)[kVals<T>], unsigned vo,

@@ -180,3 +180,24 @@ def test_errors_are_reported(nuosc_mod):
         g.fillInitValue(ipt=1)
     with pytest.raises(N.NuOscError):
         g.p3_slice([64], [0])
+
+
+def test_fp32_calrhs_every_direction(oracle_mod, nuosc_mod):
+    """The FP32 stage kernel handles two directions per lane: its RHS must
+    match the FP64 one in every direction (odd and even alike) to FP32
+    rounding of the stencil, i.e. within 64 eps32 max|field| / h."""
+    O, N = oracle_mod, nuosc_mod
+    cfg = dict(BOX64, x1=0.8, x0=-0.8, z1=0.8, z0=-0.8)
+    a = gpu_sim(N, cfg)
+    b = gpu_sim(N, cfg, precision=N.PREC_FP32)
+    a.fillInitValue(**INIT0)
+    rng = np.random.default_rng(4)
+    st = a.get_state()
+    st = (st + 1e-2 * rng.uniform(-1, 1, st.shape)).astype(np.float32).astype(np.float64)
+    r64, r32 = a.calRHS(st), b.calRHS(st)
+    eps32 = np.finfo(np.float32).eps
+    for f in range(8):
+        bound = 64 * eps32 * np.max(np.abs(st[f])) / cfg["dx"]
+        for par in (0, 1):
+            err = np.max(np.abs(r32[f][..., par::2] - r64[f][..., par::2]))
+            assert err <= bound, (f, par, err, bound)
