@@ -5,7 +5,8 @@ Fourier-mode amplification 1 + z + z^2/2 + z^3/6 (tests/test_oracle_rhs.py).
 
 Bars: 200 steps relative L-inf <= 1e-10 per field, Pee and |M0| relative
 1e-10, |P|-1 absolute 1e-12 + relative 1e-10; graph replay bitwise equal to
-eager launches; open-z column at the same bar; FP32 Pee within 1e-6 of FP64.
+eager launches; open-z column at the same bar; FP32 Pee and max |P|-1 within
+1e-6 and 1e-5 of FP64.
 """
 import numpy as np
 import pytest
@@ -87,4 +88,6 @@ def test_srk3_fp32_against_fp64(nuosc_mod):
         b.step_srk3(40)
         x, y = a.analysis(), b.analysis()
         assert abs(y.surv - x.surv) <= 1e-6
-        assert y.maxdP <= 1e-5
+        # SSP-RK3 does not preserve |P| (its amplitude error with the vacuum
+        # term on is ~1e-4 over this run, FP64 included): compare with FP64
+        assert abs(y.maxdP - x.maxdP) <= 1e-5
