@@ -187,7 +187,7 @@ def test_fp32_calrhs_every_direction(oracle_mod, nuosc_mod):
     match the FP64 one in every direction (odd and even alike) to FP32
     rounding of the stencil, i.e. within 64 eps32 max|field| / h."""
     O, N = oracle_mod, nuosc_mod
-    cfg = dict(BOX64, x1=0.8, x0=-0.8, z1=0.8, z0=-0.8)
+    cfg = dict(BOX64, x1=0.8, x0=-0.8, z1=0.8, z0=-0.8, pmo=1.0)   # vacuum term on
     a = gpu_sim(N, cfg)
     b = gpu_sim(N, cfg, precision=N.PREC_FP32)
     a.fillInitValue(**INIT0)
