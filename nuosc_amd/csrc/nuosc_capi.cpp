@@ -373,6 +373,7 @@ int nuosc_create(const nuosc_config* cfg, nuosc_sim** out) {
     s->geo.nvp = s->nvp;
     s->geo.nch = s->nch;
     s->geo.lpr = lpr;
+    s->geo.rows = rows_per_wave((int)s->esize);
     s->geo.fs = s->fs;
     s->geo.twelve_dx = 12 * c.dx;
     s->geo.twelve_dz = 12 * c.dz;

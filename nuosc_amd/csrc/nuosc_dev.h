@@ -39,6 +39,7 @@ enum AnaIdx {
 struct Geometry {
     int nx, nz, nv, nvp;
     int lpr;               // directions per chunk: one 128-byte row segment (dirs_per_chunk)
+    int rows;              // x rows per stage-kernel wavefront (rows_per_wave)
     int nch;               // direction chunks per site = ceil(nv / lpr)
     long long fs;          // elements per field
     double twelve_dx, twelve_dz;   // 12*dx, 12*dz (advection denominators)
@@ -73,9 +74,9 @@ struct StagePlan {
     int zlen, nseg, nquads, ntask, nblocks;   // z-segment length and count, x-row quads, waves, blocks
 };
 
-// x-rows per wavefront of the stage kernel: 4 rows x 16 lanes, each lane
-// loading 8 bytes (FP64: 1 direction, FP32: 2 directions) per field
-constexpr int kRows = 4;
+// x-rows per wavefront of the stage kernel (FP64 4 rows x 16 directions,
+// FP32 2 rows x 32 directions: one 128-byte row segment per field and row)
+int rows_per_wave(int esize);
 // directions per chunk = one 128-byte row segment (FP64 16, FP32 32)
 int dirs_per_chunk(int esize);
 

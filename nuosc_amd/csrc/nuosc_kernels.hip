@@ -33,9 +33,22 @@
 
 namespace nuosc {
 
-// directions per lane in the stage kernel: one 8-byte load per field
+// Stage-kernel wave shape per precision: R x rows of 64/R lanes, V adjacent
+// directions per lane (a row segment of one field is one 128-byte line).
+// FP64 4 x 16 lanes x 1; FP32 2 x 32 lanes x 1 (measured faster than
+// 4 x 16 lanes x 2: it keeps 3 waves per SIMD).
 template <typename T>
-constexpr int kVals = (int)(sizeof(double) / sizeof(T));
+struct StageShape;
+template <>
+struct StageShape<double> {
+    static constexpr int R = 4, V = 1;
+};
+template <>
+struct StageShape<float> {
+    static constexpr int R = 2, V = 1;
+};
+template <typename T>
+constexpr int kVals = StageShape<T>::V;
 
 // ---------------------------------------------------------------------------
 // helpers
@@ -157,31 +170,40 @@ struct Fields {
     __device__ __forceinline__ void st(unsigned long long b, int f, T x, unsigned vo, unsigned so) const {
         Lane<T>::st(x, rsrc(b, f), vo, so + foff(f));
     }
-    // kVals<T> adjacent values (8 bytes) per lane.  The two floats are split
-    // off through a 64-bit integer: taking the elements of the loaded vector
-    // directly makes this compiler narrow the load to its first dword and
-    // return that dword for both.
+    // kVals<T> adjacent values per lane.  Two floats are split off their
+    // 8-byte load through a 64-bit integer: taking the elements of the loaded
+    // vector directly makes this compiler narrow the load to its first dword
+    // and return that dword for both.  AUX = 2 marks a streaming access (the
+    // nt cache policy): data read or written once per stage.
+    template <int AUX = 0>
     __device__ __forceinline__ void ldv(unsigned long long b, int f, unsigned vo, unsigned so,
                                         T (&x)[kVals<T>]) const {
-        const u32x2 r = __builtin_amdgcn_raw_buffer_load_b64(rsrc(b, f), vo, so + foff(f), 0);
-        if constexpr (sizeof(T) == 8) {
-            x[0] = __builtin_bit_cast(double, r);
+        if constexpr (sizeof(T) * kVals<T> == 4) {
+            x[0] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rsrc(b, f), vo, so + foff(f), AUX));
+        } else if constexpr (sizeof(T) == 8) {
+            x[0] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rsrc(b, f), vo, so + foff(f), AUX));
         } else {
+            const u32x2 r = __builtin_amdgcn_raw_buffer_load_b64(rsrc(b, f), vo, so + foff(f), AUX);
             const unsigned long long w = __builtin_bit_cast(unsigned long long, r);
             x[0] = __builtin_bit_cast(float, (unsigned)w);
             x[1] = __builtin_bit_cast(float, (unsigned)(w >> 32));
         }
     }
+    // stores stream (nt): the next reader is the next stage, a full pass later
     __device__ __forceinline__ void stv(unsigned long long b, int f, const T (&x)[kVals<T>], unsigned vo,
                                         unsigned so) const {
-        u32x2 r;
-        if constexpr (sizeof(T) == 8) {
-            r = __builtin_bit_cast(u32x2, x[0]);
+        if constexpr (sizeof(T) * kVals<T> == 4) {
+            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x[0]), rsrc(b, f), vo, so + foff(f), 2);
         } else {
-            r[0] = __builtin_bit_cast(unsigned, x[0]);
-            r[1] = __builtin_bit_cast(unsigned, x[1]);
+            u32x2 r;
+            if constexpr (sizeof(T) == 8) {
+                r = __builtin_bit_cast(u32x2, x[0]);
+            } else {
+                r[0] = __builtin_bit_cast(unsigned, x[0]);
+                r[1] = __builtin_bit_cast(unsigned, x[1]);
+            }
+            __builtin_amdgcn_raw_buffer_store_b64(r, rsrc(b, f), vo, so + foff(f), 2);
         }
-        __builtin_amdgcn_raw_buffer_store_b64(r, rsrc(b, f), vo, so + foff(f), 0);
     }
 };
 
@@ -435,12 +457,12 @@ __device__ __forceinline__ void stage_body(const StageArgs<T, FPD>& a, int lbloc
         if constexpr (STAGE == 2 || STAGE == 3 || STAGE == 6 || STAGE == 7) {
             const unsigned long long bu = a.u.fresh_base();
 #pragma unroll
-            for (int f = 0; f < kNVar; ++f) a.u.ldv(bu, f, vo, so, uc[f]);
+            for (int f = 0; f < kNVar; ++f) a.u.template ldv<2>(bu, f, vo, so, uc[f]);
         }
         if constexpr (STAGE >= 2 && STAGE <= 4) {
             const unsigned long long bw = a.w.fresh_base();
 #pragma unroll
-            for (int f = 0; f < kNVar; ++f) a.w.ldv(bw, f, vo, so, wc[f]);
+            for (int f = 0; f < kNVar; ++f) a.w.template ldv<2>(bw, f, vo, so, wc[f]);
         }
 #pragma unroll
         for (int h = 0; h < NH; ++h)
@@ -543,16 +565,18 @@ __device__ __forceinline__ void stage_body(const StageArgs<T, FPD>& a, int lbloc
                     }
                 }
             }
-            // lane dl of a row holds the row total of value dl (16 lanes per
-            // row, 12 values used; values 12..15 fill unused record slots)
+            // lane dl of a row holds the row total of value dl >> sh (12 values
+            // used; values 12..15 fill unused record slots; lanes sharing a
+            // value store the same bits)
             const double tot = group_sum16<L, COMP>(t, te, lane);
+            constexpr int sh = (L == 16) ? 0 : (L == 32 ? 1 : 2);
             unsigned long long bp = a.mpart;
             asm volatile("" : "+s"(bp));
             const __amdgpu_buffer_rsrc_t rp =
                 __builtin_amdgcn_make_buffer_rsrc((void*)bp, 0, (int)a.mpart_bytes, kRsrcWord3);
             const unsigned rec = ((unsigned)(rst - rmin) * (unsigned)nz * nch + (unsigned)ch) * (kMomRec * 8);
             const unsigned srec = (mbase + j) * (unsigned)(nch * kMomRec * 8);
-            Lane<double>::st(tot, rp, rec + (unsigned)dl * 8, srec);
+            Lane<double>::st(tot, rp, rec + (unsigned)(dl >> sh) * 8, srec);
         }
 #pragma unroll
         for (int f = 0; f < kNVar; ++f)
@@ -601,13 +625,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
     stage_body<T, R, FPD, STAGE, COMP>(a, blk, wib, slab[wib]);
 }
 
-int dirs_per_chunk(int esize) { return (kWave / kRows) * (8 / esize); }
+int rows_per_wave(int esize) { return esize == 8 ? StageShape<double>::R : StageShape<float>::R; }
+int dirs_per_chunk(int esize) {
+    return esize == 8 ? (kWave / StageShape<double>::R) * StageShape<double>::V
+                      : (kWave / StageShape<float>::R) * StageShape<float>::V;
+}
 
 StagePlan plan_stage(const Geometry& g, int num_cus) {
     // enough waves to give every SIMD several (>= 12 per CU), segments long
     // enough that the window warm-up stays a small overhead
     StagePlan p{};
-    const int R = kRows;
+    const int R = g.rows;
     const long long quads = (g.nx + R - 1) / R;
     const long long rows = quads * g.nch;
     const long long want = (long long)num_cus * 48;
@@ -624,7 +652,7 @@ StagePlan plan_stage(const Geometry& g, int num_cus) {
 
 StagePlan plan_stage_zlen(const Geometry& g, int zlen) {
     StagePlan p{};
-    const int R = kRows;
+    const int R = g.rows;
     const long long quads = (g.nx + R - 1) / R;
     p.zlen = std::max(1, std::min(zlen, g.nz));
     p.nseg = (g.nz + p.zlen - 1) / p.zlen;
@@ -692,7 +720,7 @@ static hipError_t launch_stage_fpd(int stage, const Geometry& g, const Physics& 
                                    double cw, double cy, hipStream_t s, double ca, double cb) {
     const StageArgs<T, FPD> a = make_stage_args<T, FPD>(g, ph, pl, yin, u, w, yout, mtot, mpart, cw, cy, ca, cb);
     constexpr bool COMP = sizeof(T) == 4;
-    constexpr int R = kRows;
+    constexpr int R = StageShape<T>::R;
     if (g.lpr != dirs_per_chunk((int)sizeof(T))) return hipErrorInvalidValue;
     dim3 grid(pl.nblocks), block(256);
     switch (stage) {
@@ -879,7 +907,7 @@ void batch_free(BatchTables& t) {
 
 template <typename T, int FPD>
 static hipError_t batch_step_t(const BatchTables& t, hipStream_t s) {
-    constexpr int R = kRows;
+    constexpr int R = StageShape<T>::R;
     constexpr bool COMP = sizeof(T) == 4;
     const dim3 grid(t.nblocks), block(256), tgrid(t.tblocks);
     for (int st = 1; st <= 4; ++st) {
@@ -897,7 +925,7 @@ static hipError_t batch_step_t(const BatchTables& t, hipStream_t s) {
 
 template <typename T, int FPD>
 static hipError_t batch_stage_t(const BatchTables& t, int st, hipStream_t s) {
-    constexpr int R = kRows;
+    constexpr int R = StageShape<T>::R;
     constexpr bool COMP = sizeof(T) == 4;
     const dim3 grid(t.nblocks), block(256);
     const StageArgs<T, FPD>* a = (const StageArgs<T, FPD>*)t.args[st - 1];
