@@ -29,7 +29,7 @@ struct nuosc_sim {
     nuosc_config cfg{};
     // geometry (CartGrid.cpp:191-208)
     int nx = 0, nz = 0, nv = 0, nvp = 0, nch = 0;
-    long long fs = 0;
+    long long copy_elems = 0;   // elements per state copy
     double dt = 0, ds_L = 0, ct = 0, st = 0;
     double phy_time = 0;
     double n_nue0[2] = {0, 0};
@@ -86,11 +86,6 @@ int fail(nuosc_sim* s, int code, const std::string& msg) {
 
 bool is64(const nuosc_sim* s) { return s->cfg.precision == NUOSC_PREC_FP64; }
 
-template <typename T>
-T* field_ptr(void* base, const nuosc_sim* s, int f) {
-    return static_cast<T*>(base) + (long long)f * s->fs;
-}
-
 void drop_graph(nuosc_sim* s) {
     if (s->graph_exec) hipGraphExecDestroy(s->graph_exec);
     if (s->graph) hipGraphDestroy(s->graph);
@@ -131,12 +126,17 @@ int enqueue_step(nuosc_sim* s, hipEvent_t* ev = nullptr) {
     const Geometry& g = s->geo;
     const bool comp = sizeof(T) == 4;
     double* P = s->mom_part;
-    // open z boundary: ghost rows of each stage input at the stage time
-    // t, t + dt/2, t + dt/2, t + dt (t read from the device copy)
+    // ghost rows of each stage input: open z ends at the stage time t,
+    // t + dt/2, t + dt/2, t + dt (t read from the device copy), then the
+    // periodic x ghost rows (whole stored rows, z ghosts included)
     const bool open = g.bc == NUOSC_BC_OPEN_Z;
     auto ghosts = [&](T* y, double cdt) {
-        return open ? launch_open_ghosts<T>(g, y, s->d_g0, s->d_g0 + s->nv, s->inflow, s->d_time, cdt, s->stream)
-                    : hipSuccess;
+        if (open) {
+            const hipError_t e =
+                launch_open_ghosts<T>(g, y, s->d_g0, s->d_g0 + s->nv, s->inflow, s->d_time, cdt, s->stream);
+            if (e != hipSuccess) return e;
+        }
+        return launch_xghosts<T>(g, y, s->stream);
     };
     auto mark = [&](int k) { return ev ? hipEventRecord(ev[k], s->stream) : hipSuccess; };
     HIPCK(s, ghosts(u, 0.0));
@@ -182,8 +182,12 @@ int enqueue_step_srk3(nuosc_sim* s) {
     double* P = s->mom_part;
     const bool open = g.bc == NUOSC_BC_OPEN_Z;
     auto ghosts = [&](T* y, double cdt) {
-        return open ? launch_open_ghosts<T>(g, y, s->d_g0, s->d_g0 + s->nv, s->inflow, s->d_time, cdt, s->stream)
-                    : hipSuccess;
+        if (open) {
+            const hipError_t e =
+                launch_open_ghosts<T>(g, y, s->d_g0, s->d_g0 + s->nv, s->inflow, s->d_time, cdt, s->stream);
+            if (e != hipSuccess) return e;
+        }
+        return launch_xghosts<T>(g, y, s->stream);
     };
     HIPCK(s, ghosts(u, 0.0));
     HIPCK(s, launch_stage<T>(5, g, s->ph, s->plan, u, nullptr, nullptr, ya, s->mom_u, P, 0.0, dt, s->stream));
@@ -227,15 +231,19 @@ int analysis_sums(nuosc_sim* s, double* h) {
     return NUOSC_OK;
 }
 
-// open z boundary: ghost rows of the state u at the current time
+// ghost rows of state y at the current time: open z ends (if any), then the
+// periodic x ghost rows
 int fill_state_ghosts(nuosc_sim* s, void* y) {
-    if (s->geo.bc != NUOSC_BC_OPEN_Z) return NUOSC_OK;
-    if (is64(s))
-        HIPCK(s, launch_open_ghosts<double>(s->geo, (double*)y, s->d_g0, s->d_g0 + s->nv, s->inflow, s->d_time, 0.0,
-                                            s->stream));
-    else
-        HIPCK(s, launch_open_ghosts<float>(s->geo, (float*)y, s->d_g0, s->d_g0 + s->nv, s->inflow, s->d_time, 0.0,
-                                           s->stream));
+    if (s->geo.bc == NUOSC_BC_OPEN_Z) {
+        if (is64(s))
+            HIPCK(s, launch_open_ghosts<double>(s->geo, (double*)y, s->d_g0, s->d_g0 + s->nv, s->inflow, s->d_time,
+                                                0.0, s->stream));
+        else
+            HIPCK(s, launch_open_ghosts<float>(s->geo, (float*)y, s->d_g0, s->d_g0 + s->nv, s->inflow, s->d_time,
+                                               0.0, s->stream));
+    }
+    if (is64(s)) HIPCK(s, launch_xghosts<double>(s->geo, (double*)y, s->stream));
+    else HIPCK(s, launch_xghosts<float>(s->geo, (float*)y, s->stream));
     return NUOSC_OK;
 }
 
@@ -323,10 +331,20 @@ int nuosc_create(const nuosc_config* cfg, nuosc_sim** out) {
     s->nvp = (s->nv + align - 1) / align * align;
     const int lpr = dirs_per_chunk((int)s->esize);
     s->nch = (s->nv + lpr - 1) / lpr;
-    s->fs = (long long)(s->nx + 1) * nzs * s->nvp;   // + the stage kernels' spare x row
-    if (s->fs * (long long)s->esize >= (1ll << 32) - 1) {
-        delete s;
-        return fail(nullptr, NUOSC_EINVAL, "one field must stay below 4 GiB (nx*nz*nv too large for one GPU handle)");
+    {
+        Geometry gq{};
+        gq.nx = s->nx;
+        gq.nz = s->nz;
+        gq.nch = s->nch;
+        gq.nxs = s->nx + 2 * kXg + 1;
+        gq.xrow = (long long)nzs * kNVar * s->nvp;
+        if (!buffer_addressable(gq, (int)s->esize)) {
+            delete s;
+            return fail(nullptr, NUOSC_EINVAL,
+                        "grid too large for one GPU handle: a few stored x rows (nz * nv) or the moment records "
+                        "(nx * nz * nv) exceed 4 GiB");
+        }
+        s->copy_elems = copy_elems(gq);
     }
     // nuosc_class.cpp:461-464, nuosc_class.h:120-122
     s->ds_L = c.dx * c.dz / (c.z1 - c.z0) / (c.x1 - c.x0);
@@ -347,7 +365,7 @@ int nuosc_create(const nuosc_config* cfg, nuosc_sim** out) {
     if (hipEventCreate(&s->ev0) != hipSuccess || hipEventCreate(&s->ev1) != hipSuccess)
         return bail(NUOSC_EHIP, "event creation failed");
 
-    const size_t state_bytes = (size_t)kNVar * s->fs * s->esize;
+    const size_t state_bytes = (size_t)s->copy_elems * s->esize;
     const size_t part_bytes = (size_t)(s->nx + 1) * s->nz * s->nch * kMomRec * sizeof(double);
     const size_t mom_bytes = (size_t)s->nx * s->nz * kMomStride * sizeof(double);
     void** states[4] = {&s->st_u, &s->st_w, &s->st_ya, &s->st_yb};
@@ -374,7 +392,8 @@ int nuosc_create(const nuosc_config* cfg, nuosc_sim** out) {
     s->geo.nch = s->nch;
     s->geo.lpr = lpr;
     s->geo.rows = rows_per_wave((int)s->esize);
-    s->geo.fs = s->fs;
+    s->geo.nxs = s->nx + 2 * kXg + 1;
+    s->geo.xrow = (long long)nzs * kNVar * s->nvp;
     s->geo.twelve_dx = 12 * c.dx;
     s->geo.twelve_dz = 12 * c.dz;
     s->geo.ko_x = -c.ko / c.dx / 16.0;  // nuosc_class.cpp:120-121
@@ -720,6 +739,10 @@ int nuosc_autotune(nuosc_sim* s, int trials, int* best_zlen, double* best_ms) {
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
         return fail(s, NUOSC_EHIP, "event creation failed");
+    {
+        const int rc = fill_state_ghosts(s, s->st_u);   // stage 1 reads u
+        if (rc) return rc;
+    }
     const StagePlan keep = s->plan;
     int best = keep.zlen;
     float bms = 1e30f;
@@ -774,8 +797,8 @@ int nuosc_calrhs(nuosc_sim* s, const double* in, double* out) {
     const size_t fe = (size_t)s->nx * s->nz * s->nv;
     for (int f = 0; f < kNVar; ++f) {
         HIPCK(s, hipMemcpyAsync(s->d_tmp, in + f * fe, fbytes, hipMemcpyHostToDevice, s->stream));
-        if (is64(s)) HIPCK(s, launch_unpack<double>(s->geo, s->d_tmp, field_ptr<double>(s->st_ya, s, f), s->stream));
-        else HIPCK(s, launch_unpack<float>(s->geo, s->d_tmp, field_ptr<float>(s->st_ya, s, f), s->stream));
+        if (is64(s)) HIPCK(s, launch_unpack<double>(s->geo, s->d_tmp, (double*)s->st_ya, f, s->stream));
+        else HIPCK(s, launch_unpack<float>(s->geo, s->d_tmp, (float*)s->st_ya, f, s->stream));
     }
     rc = fill_state_ghosts(s, s->st_ya);
     if (rc) return rc;
@@ -791,8 +814,8 @@ int nuosc_calrhs(nuosc_sim* s, const double* in, double* out) {
                                      (float*)s->st_yb, s->mom_a, nullptr, 0.0, 0.0, s->stream));
     }
     for (int f = 0; f < kNVar; ++f) {
-        if (is64(s)) HIPCK(s, launch_pack<double>(s->geo, field_ptr<double>(s->st_yb, s, f), s->d_tmp, s->stream));
-        else HIPCK(s, launch_pack<float>(s->geo, field_ptr<float>(s->st_yb, s, f), s->d_tmp, s->stream));
+        if (is64(s)) HIPCK(s, launch_pack<double>(s->geo, (double*)s->st_yb, f, s->d_tmp, s->stream));
+        else HIPCK(s, launch_pack<float>(s->geo, (float*)s->st_yb, f, s->d_tmp, s->stream));
         HIPCK(s, hipMemcpyAsync(out + f * fe, s->d_tmp, fbytes, hipMemcpyDeviceToHost, s->stream));
     }
     HIPCK(s, hipStreamSynchronize(s->stream));
@@ -839,8 +862,8 @@ int nuosc_download_field(nuosc_sim* s, int f, double* host) {
     const size_t bytes = sizeof(double) * (size_t)s->nx * s->nz * s->nv;
     rc = ensure_tmp(s, bytes);
     if (rc) return rc;
-    if (is64(s)) HIPCK(s, launch_pack<double>(s->geo, field_ptr<double>(s->st_u, s, f), s->d_tmp, s->stream));
-    else HIPCK(s, launch_pack<float>(s->geo, field_ptr<float>(s->st_u, s, f), s->d_tmp, s->stream));
+    if (is64(s)) HIPCK(s, launch_pack<double>(s->geo, (double*)s->st_u, f, s->d_tmp, s->stream));
+    else HIPCK(s, launch_pack<float>(s->geo, (float*)s->st_u, f, s->d_tmp, s->stream));
     HIPCK(s, hipMemcpyAsync(host, s->d_tmp, bytes, hipMemcpyDeviceToHost, s->stream));
     HIPCK(s, hipStreamSynchronize(s->stream));
     return NUOSC_OK;
@@ -857,10 +880,10 @@ int nuosc_download_field_ghosted(nuosc_sim* s, int f, double* host) {
     rc = fill_state_ghosts(s, s->st_u);
     if (rc) return rc;
     if (is64(s))
-        HIPCK(s, launch_ghosted<double>(s->geo, s->cfg.gx, s->cfg.gz, field_ptr<double>(s->st_u, s, f), s->d_tmp,
+        HIPCK(s, launch_ghosted<double>(s->geo, s->cfg.gx, s->cfg.gz, (double*)s->st_u, f, s->d_tmp,
                                         s->stream));
     else
-        HIPCK(s, launch_ghosted<float>(s->geo, s->cfg.gx, s->cfg.gz, field_ptr<float>(s->st_u, s, f), s->d_tmp,
+        HIPCK(s, launch_ghosted<float>(s->geo, s->cfg.gx, s->cfg.gz, (float*)s->st_u, f, s->d_tmp,
                                        s->stream));
     HIPCK(s, hipMemcpyAsync(host, s->d_tmp, bytes, hipMemcpyDeviceToHost, s->stream));
     HIPCK(s, hipStreamSynchronize(s->stream));
@@ -875,8 +898,8 @@ int nuosc_upload_field(nuosc_sim* s, int f, const double* host) {
     rc = ensure_tmp(s, bytes);
     if (rc) return rc;
     HIPCK(s, hipMemcpyAsync(s->d_tmp, host, bytes, hipMemcpyHostToDevice, s->stream));
-    if (is64(s)) HIPCK(s, launch_unpack<double>(s->geo, s->d_tmp, field_ptr<double>(s->st_u, s, f), s->stream));
-    else HIPCK(s, launch_unpack<float>(s->geo, s->d_tmp, field_ptr<float>(s->st_u, s, f), s->stream));
+    if (is64(s)) HIPCK(s, launch_unpack<double>(s->geo, s->d_tmp, (double*)s->st_u, f, s->stream));
+    else HIPCK(s, launch_unpack<float>(s->geo, s->d_tmp, (float*)s->st_u, f, s->stream));
     HIPCK(s, hipStreamSynchronize(s->stream));
     s->have_moments = false;
     return NUOSC_OK;
@@ -1138,7 +1161,7 @@ int nuosc_estimate_device_bytes(const nuosc_config* c, long long* bytes) {
     const long long align = 128 / esize, nvp = (nv + align - 1) / align * align;
     const long long lpr = dirs_per_chunk(esize), nch = (nv + lpr - 1) / lpr;
     const long long nzs = nz + (c->bc == NUOSC_BC_OPEN_Z ? 2 * c->gz : 0);
-    long long b = 4ll * kNVar * (nx + 1) * nzs * nvp * esize;   // u, w, yA, yB (+ spare x row)
+    long long b = 4ll * (nx + 2 * kXg + 1) * nzs * kNVar * nvp * esize;   // u, w, yA, yB (+ x ghost, spare rows)
     b += (nx + 1) * nz * nch * kMomRec * 8 + 3ll * nx * nz * kMomStride * 8;  // moments
     b += 4 * nv * 8 + 2 * nv * 8 + 2048ll * kNAna * 16 + kNAna * 8;    // v-grid, G0, analysis
     b += 2 * nz * 8;                                                   // init tables

@@ -2,21 +2,29 @@
 // and the HIP kernels (nuosc_kernels.hip).  Not part of the public ABI.
 //
 // Device data layout (one state copy):
-//   8 fields (ee, mm, emr, emi, bee, bmm, bemr, bemi) one after another, each
-//   field [nx][nzs][nvp] with v fastest; nvp = nv rounded up so that every
-//   site row starts on a 128-byte boundary.  Periodic runs store no ghost
-//   band (nzs = nz): the boundary is an index wrap inside the stencil loads.
-//   Runs with an open z boundary store zoff = gz ghost rows at each z end
-//   (nzs = nz + 2 gz, interior row j at stored row j + zoff), refilled before
-//   every stage; x stays periodic by index wrap.
+//   site records of 8 fields (ee, mm, emr, emi, bee, bmm, bemr, bemi), each
+//   field a row of nvp directions (v fastest; nvp = nv rounded up so that
+//   every field row starts on a 128-byte boundary):
+//     copy[xs][zs][f][nvp],  xs = i + kXg (stored x row), zs = j + zoff
+//   so the 8 fields of a site are one contiguous 8*nvp block and a stored x
+//   row (nzs site records) is contiguous.  Stored x rows: kXg = 2 periodic
+//   ghost rows at each end (copies of interior rows nx-2, nx-1 and 0, 1,
+//   refreshed before every stage from the interior) and one spare row at
+//   xs = nx + 2 kXg, where stage-kernel lanes past the last x row store
+//   instead of branching; nxs = nx + 2 kXg + 1.  The x-neighbours of every
+//   wave are therefore one contiguous address range (a few stored rows, one
+//   buffer descriptor) and a z-march stays inside a few memory pages.
+//   Periodic runs store no z ghost rows (nzs = nz; the z boundary is an index
+//   wrap inside the stencil loads).  Runs with an open z boundary store
+//   zoff = gz ghost rows at each z end (nzs = nz + 2 gz), refilled before
+//   every stage.
 //   Velocity moments: 12 doubles {A0..A3, B0..B3, C0..C3} per site with
 //   A = sum w u (bemr-emr), B = sum w u (bemi+emi), C = sum w u
 //   (bee-bmm-ee+mm), u = (1, vx, vy, vz).  The stage kernels write partial
 //   records [nx*nz sites][nch chunks][16] (12 used, each summed over the lpr
-//   directions of one chunk); k_moment_totals adds them in chunk order into
-//   totals [nx*nz sites][16], which the next stage reads.
-//   Every field and the partial records carry one spare x row (index nx):
-//   stage-kernel lanes past the last x row store there instead of branching.
+//   directions of one chunk) plus one spare x row of records; k_moment_totals
+//   adds them in chunk order into totals [nx*nz sites][16], which the next
+//   stage reads.
 #pragma once
 #include <hip/hip_runtime_api.h>
 #include <cstdint>
@@ -30,6 +38,7 @@ constexpr int kMomStride = 16;   // doubles per per-site moment total record (12
 constexpr int kMomRec = 16;      // doubles per (site, chunk) partial moment record (12 used)
 constexpr int kNMom = 12;
 constexpr int kNAna = 12;        // analysis accumulators (see AnaIdx)
+constexpr int kXg = 2;           // stored periodic ghost x rows at each end
 
 enum AnaIdx {
     ANA_SURV = 0, ANA_SURVB, ANA_AVGP, ANA_AVGPB, ANA_AM01, ANA_AM02, ANA_AM03,
@@ -41,13 +50,26 @@ struct Geometry {
     int lpr;               // directions per chunk: one 128-byte row segment (dirs_per_chunk)
     int rows;              // x rows per stage-kernel wavefront (rows_per_wave)
     int nch;               // direction chunks per site = ceil(nv / lpr)
-    long long fs;          // elements per field
+    int nxs;               // stored x rows: nx + 2 kXg + 1 (ghost rows and the spare row)
+    long long xrow;        // elements per stored x row: nzs * 8 * nvp
     double twelve_dx, twelve_dz;   // 12*dx, 12*dz (advection denominators)
     double ko_x, ko_z;     // -ko/dx/16, -ko/dz/16
     const double *vx, *vy, *vz, *vw;   // device, length nv
     int bc;                // 0 periodic, 1 open in z
     int nzs, zoff;         // stored z rows per x row, stored row of interior z = 0
 };
+
+// element offset of field f, stored z row js, direction 0 of interior x row i
+// (i = -kXg .. nx + kXg; nx + kXg is the spare row)
+__host__ __device__ inline long long elem_off(const Geometry& g, long long i, long long js, int f) {
+    return ((i + kXg) * g.nzs + js) * (long long)(kNVar * g.nvp) + (long long)f * g.nvp;
+}
+// element offset of direction 0, field 0 of interior site (i * nz + j)
+__host__ __device__ inline long long site_off(const Geometry& g, long long site) {
+    return elem_off(g, site / g.nz, site % g.nz + g.zoff, 0);
+}
+// elements per state copy
+inline long long copy_elems(const Geometry& g) { return (long long)g.nxs * g.xrow; }
 
 // Inflow data of the open z boundary: the ipt-0 pulse
 // a = eps0 exp(-(z - vz t - zc)^2 / (2 sigma^2)) (on = 0: a = 0).
@@ -83,14 +105,21 @@ int dirs_per_chunk(int esize);
 StagePlan plan_stage(const Geometry& g, int num_cus);
 // the same decomposition with a given z-segment length
 StagePlan plan_stage_zlen(const Geometry& g, int zlen);
-// The stage kernel addresses each field through a buffer descriptor: one
-// field must span fewer than 4 GiB.
+// The stage kernel addresses the stored x rows of one wavefront (rows + 5 of
+// them) and the moment records through buffer descriptors: each range must
+// span fewer than 4 GiB.
 bool buffer_addressable(const Geometry& g, int esize);
 
 template <typename T>
 hipError_t launch_stage(int stage, const Geometry& g, const Physics& ph, const StagePlan& pl,
                         const T* yin, const T* u, T* w, T* yout, const double* mtot, double* mpart,
                         double cw, double cy, hipStream_t s, double ca = 0.0, double cb = 0.0);
+
+// periodic x ghost rows of a state copy: stored rows 0, 1 <- interior rows
+// nx-2, nx-1 and rows nx+2, nx+3 <- interior 0, 1 (whole stored rows, z
+// ghost rows included).  Run on every stage input before the stage.
+template <typename T>
+hipError_t launch_xghosts(const Geometry& g, T* y, hipStream_t s);
 
 // partial records -> per-site totals (compensated sums for the FP32 form)
 hipError_t launch_moment_totals(const Geometry& g, bool compensated, const double* part, double* tot,
@@ -114,17 +143,17 @@ template <typename T>
 hipError_t launch_analysis(const Geometry& g, const T* y, const double* G0, const double* G0b,
                            double* part, double* out, hipStream_t s);
 
-// Reference-layout copy of one field with its ghost band filled by the
-// periodic rule (corners zero), as FP64: dst[(nx+2gx)*(nz+2gz)*nv].
+// Reference-layout copy of field f of state y with its ghost band filled by
+// the periodic rule (corners zero), as FP64: dst[(nx+2gx)*(nz+2gz)*nv].
 template <typename T>
-hipError_t launch_ghosted(const Geometry& g, int gx, int gz, const T* field, double* dst,
+hipError_t launch_ghosted(const Geometry& g, int gx, int gz, const T* y, int f, double* dst,
                           hipStream_t s);
 
-// Interior field <-> FP64 dense [nx][nz][nv].
+// Interior of field f of state y <-> FP64 dense [nx][nz][nv].
 template <typename T>
-hipError_t launch_pack(const Geometry& g, const T* field, double* dst, hipStream_t s);
+hipError_t launch_pack(const Geometry& g, const T* y, int f, double* dst, hipStream_t s);
 template <typename T>
-hipError_t launch_unpack(const Geometry& g, const double* src, T* field, hipStream_t s);
+hipError_t launch_unpack(const Geometry& g, const double* src, T* y, int f, hipStream_t s);
 
 // P3 = (ee-mm)/G0 at x indices xi[sx] and v indices vi[sv]: dst[sx][nz][sv].
 template <typename T>
@@ -146,14 +175,17 @@ struct BatchMember {
 struct BatchTables {
     void* args[4] = {nullptr, nullptr, nullptr, nullptr};  // per-stage StageArgs arrays
     void* tot[4] = {nullptr, nullptr, nullptr, nullptr};   // per-stage moment-total arrays
+    void* xg[4] = {nullptr, nullptr, nullptr, nullptr};    // per-stage x-ghost refresh arrays
     int* bmap = nullptr;   // logical block -> (member, block of member)
     int* tmap = nullptr;
-    int nblocks = 0, tblocks = 0, fpd = 1, esize = 8;
+    int* xmap = nullptr;
+    int nblocks = 0, tblocks = 0, xblocks = 0, esize = 8;
 };
 hipError_t batch_build(BatchTables& t, const std::vector<BatchMember>& members, int esize, hipStream_t s);
 void batch_free(BatchTables& t);
 hipError_t batch_step(const BatchTables& t, hipStream_t s);
-// one stage kernel of every member (no moment totals): autotuner timing
+// one stage kernel of every member (no ghost refresh, no moment totals):
+// autotuner timing
 hipError_t batch_stage(const BatchTables& t, int stage, hipStream_t s);
 
 // Open z boundary: fill the zoff ghost rows at both z ends of state y for

@@ -1,15 +1,18 @@
 // nuosc_kernels.hip -- CDNA4 (gfx950) kernels of the nuosc RK4 time stepper.
 //
-// The hot path is k_stage: one launch per RK4 stage.  Each wavefront owns one
-// x-row i, one 64-direction chunk of v (lane = direction) and one z-segment,
-// and marches along z keeping a 5-point z-window of all 8 fields in registers
-// (every value of the row is loaded once).  The x-neighbours i-2..i+2 are
-// plain coalesced loads that hit in L2: a block->tile remap keeps the waves of
-// adjacent x-rows of the same (z-segment, v-chunk) on one XCD at the same
-// time.  The velocity moments needed by the interaction term (a reduction
-// over ALL directions of a site) are produced by the previous stage's
-// epilogue as per-(site, chunk) partial sums, so no separate pass over the
-// state is needed; the consumer adds the partials of a site in chunk order.
+// The hot path is k_stage: one launch per RK4 stage.  Each wavefront owns R
+// adjacent x-rows, one 128-byte direction chunk of v and one z-segment, and
+// marches along z keeping a 5-point z-window of all 8 fields in registers
+// (every value of its rows is loaded once); the x-neighbours come from a
+// small LDS slab filled with the wave's own rows and 4 halo rows, which the
+// neighbouring waves of the block load as their own rows at the same time
+// (L2 hits).  The state is stored site-interleaved ([x][z][field][v]) with
+// periodic ghost x rows, so the rows a wave touches are one contiguous range
+// under one buffer descriptor and a z-march walks through few memory pages.
+// The velocity moments needed by the interaction term (a reduction over ALL
+// directions of a site) are produced by the previous stage's epilogue as
+// per-(site, chunk) partial sums, so no separate pass over the state is
+// needed; the consumer adds the partials of a site in chunk order.
 //
 // RK4 is streamed with four state copies u, w, yA, yB (w accumulates the
 // final combination), i.e. 16 passes over the 8 fields per step:
@@ -56,12 +59,6 @@ constexpr int kVals = StageShape<T>::V;
 __device__ __forceinline__ int wrapi(int k, int n) {
     k %= n;
     return k < 0 ? k + n : k;
-}
-
-// stored site row of interior site (i*nz + j): i*nzs + j + zoff
-__device__ __forceinline__ long long sto(const Geometry& g, long long site) {
-    if (g.zoff == 0) return site;
-    return (site / g.nz) * g.nzs + site % g.nz + g.zoff;
 }
 
 // Blocks are dealt to the 8 XCDs round-robin (block b runs on XCD b % 8).
@@ -139,19 +136,17 @@ struct Lane<float> {
     }
 };
 
-// The 8 fields of one state copy; field f starts at base + f*fs.  Groups of
-// FPD consecutive fields share one buffer descriptor (a group spans fewer
-// than 4 GiB); the field's offset inside its group goes into the scalar
-// offset together with the site offset.  Descriptors are rebuilt from the
-// 64-bit base at each use (a few scalar instructions): kept live across the
-// z loop, the descriptors of four state copies overflow the scalar register
-// file.
-template <typename T, int FPD>
-struct Fields {
-    unsigned long long base;   // address of field 0
-    unsigned long long group_stride;  // FPD * field bytes
-    unsigned group_bytes;
-    unsigned field_bytes;
+// The stored x rows of one state copy that one wavefront reads or writes:
+// rows xs0 .. xs0 + nrow - 1 under one buffer descriptor (fewer than 4 GiB).
+// Offsets: lane's x row and direction in voffset, z row and field in the
+// scalar offset.  The descriptor is rebuilt from the 64-bit base at each use
+// (a few scalar instructions): kept live across the z loop, the descriptors
+// of four state copies crowd the scalar register file.
+template <typename T>
+struct Rows {
+    unsigned long long base;   // address of stored x row xs0
+    unsigned bytes;            // nrow stored x rows
+    unsigned rowb;             // bytes of one field row (nvp directions)
     // base address with an empty asm barrier: the compiler cannot hoist the
     // descriptors built from it out of the loop
     __device__ __forceinline__ unsigned long long fresh_base() const {
@@ -159,17 +154,10 @@ struct Fields {
         asm volatile("" : "+s"(b));
         return b;
     }
-    __device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(unsigned long long b, int f) const {
-        return __builtin_amdgcn_make_buffer_rsrc((void*)(b + (unsigned long long)(f / FPD) * group_stride), 0,
-                                                 (int)group_bytes, kRsrcWord3);
+    __device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(unsigned long long b) const {
+        return __builtin_amdgcn_make_buffer_rsrc((void*)b, 0, (int)bytes, kRsrcWord3);
     }
-    __device__ __forceinline__ unsigned foff(int f) const { return (unsigned)(f % FPD) * field_bytes; }
-    __device__ __forceinline__ T ld(unsigned long long b, int f, unsigned vo, unsigned so) const {
-        return Lane<T>::ld(rsrc(b, f), vo, so + foff(f));
-    }
-    __device__ __forceinline__ void st(unsigned long long b, int f, T x, unsigned vo, unsigned so) const {
-        Lane<T>::st(x, rsrc(b, f), vo, so + foff(f));
-    }
+    __device__ __forceinline__ unsigned foff(int f) const { return (unsigned)f * rowb; }
     // kVals<T> adjacent values per lane.  Two floats are split off their
     // 8-byte load through a 64-bit integer: taking the elements of the loaded
     // vector directly makes this compiler narrow the load to its first dword
@@ -179,11 +167,11 @@ struct Fields {
     __device__ __forceinline__ void ldv(unsigned long long b, int f, unsigned vo, unsigned so,
                                         T (&x)[kVals<T>]) const {
         if constexpr (sizeof(T) * kVals<T> == 4) {
-            x[0] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rsrc(b, f), vo, so + foff(f), AUX));
+            x[0] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rsrc(b), vo, so + foff(f), AUX));
         } else if constexpr (sizeof(T) == 8) {
-            x[0] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rsrc(b, f), vo, so + foff(f), AUX));
+            x[0] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rsrc(b), vo, so + foff(f), AUX));
         } else {
-            const u32x2 r = __builtin_amdgcn_raw_buffer_load_b64(rsrc(b, f), vo, so + foff(f), AUX);
+            const u32x2 r = __builtin_amdgcn_raw_buffer_load_b64(rsrc(b), vo, so + foff(f), AUX);
             const unsigned long long w = __builtin_bit_cast(unsigned long long, r);
             x[0] = __builtin_bit_cast(float, (unsigned)w);
             x[1] = __builtin_bit_cast(float, (unsigned)(w >> 32));
@@ -193,7 +181,7 @@ struct Fields {
     __device__ __forceinline__ void stv(unsigned long long b, int f, const T (&x)[kVals<T>], unsigned vo,
                                         unsigned so) const {
         if constexpr (sizeof(T) * kVals<T> == 4) {
-            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x[0]), rsrc(b, f), vo, so + foff(f), 2);
+            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x[0]), rsrc(b), vo, so + foff(f), 2);
         } else {
             u32x2 r;
             if constexpr (sizeof(T) == 8) {
@@ -202,7 +190,7 @@ struct Fields {
                 r[0] = __builtin_bit_cast(unsigned, x[0]);
                 r[1] = __builtin_bit_cast(unsigned, x[1]);
             }
-            __builtin_amdgcn_raw_buffer_store_b64(r, rsrc(b, f), vo, so + foff(f), 2);
+            __builtin_amdgcn_raw_buffer_store_b64(r, rsrc(b), vo, so + foff(f), 2);
         }
     }
 };
@@ -224,10 +212,10 @@ typedef const __attribute__((address_space(4))) double cdouble;
 // operations complete in order).  Moments of z+1 are prefetched during step
 // z.
 // ---------------------------------------------------------------------------
-template <typename T, int FPD>
+template <typename T>
 struct StageArgs {
     Geometry g;
-    Fields<T, FPD> yin, u, w, yout;
+    unsigned long long yin, u, w, yout;   // state copies (address of stored x row 0)
     unsigned long long mtot;  // moment totals of yin: [site][kMomStride] doubles
     unsigned mtot_bytes;
     unsigned long long mpart; // partial moments of yout: [site][nch][kMomRec] (+ one spare x row)
@@ -308,8 +296,8 @@ struct Slab {
 // x-row quads, z segment), chunk fastest; wave wib of the block takes quad
 // 4 * group + wib.  Lane (g, dl) of a wave handles x row i0 + g and the V
 // adjacent directions ch * D + dl * V + e, e < V.
-template <typename T, int R, int FPD, int STAGE, bool COMP>
-__device__ __forceinline__ void stage_body(const StageArgs<T, FPD>& a, int lblock, int wib, Slab<T, R>& slab) {
+template <typename T, int R, int STAGE, bool COMP>
+__device__ __forceinline__ void stage_body(const StageArgs<T>& a, int lblock, int wib, Slab<T, R>& slab) {
     constexpr int V = kVals<T>;    // directions per lane (one 8-byte load)
     constexpr int L = kWave / R;   // lanes per row
     constexpr int D = L * V;       // directions per row chunk (one 128-byte line)
@@ -319,12 +307,14 @@ __device__ __forceinline__ void stage_body(const StageArgs<T, FPD>& a, int lbloc
     const int lane = threadIdx.x & 63;
     const int nx = a.g.nx, nz = a.g.nz, nch = a.g.nch, nv = a.g.nv;
     const int nzs = a.g.nzs, zoff = a.g.zoff;             // stored rows per x row, ghost rows below z = 0
-    const unsigned rowb = (unsigned)a.g.nvp * sizeof(T);  // bytes per site row
+    const unsigned rowb = (unsigned)a.g.nvp * sizeof(T);  // bytes per field row
+    const unsigned siteb = rowb * kNVar;                  // bytes per site record (8 field rows)
+    const unsigned xrowb = (unsigned)nzs * siteb;         // bytes per stored x row
     // The four waves of a block sweep 4R adjacent x rows of one chunk side by
     // side, so most halo rows a wave reads are rows its neighbours in the
     // block read at about the same time (L2 hits).  Chunk fastest across
     // blocks: neighbouring blocks, which the XCD remap keeps on one XCD, read
-    // adjacent 128-byte segments of the same site rows.
+    // adjacent 128-byte segments of the same site records.
     const int ch = lblock % nch;
     const int rest = lblock / nch;
     const int nqg = (a.nquads + 3) >> 2;
@@ -338,36 +328,34 @@ __device__ __forceinline__ void stage_body(const StageArgs<T, FPD>& a, int lbloc
     const bool row_ok = r < nx;
     const bool act = row_ok && p * V < nv;   // nv is a multiple of V
     const int pl = min(p, nv / V - 1);       // loads stay inside the directions
-    const int rw = wrapi(r, nx);
 
-    // lowest referenced row = scalar base; per-lane row deltas (>= 0) in the
-    // vector offsets
-    int rmin = wrapi(i0 - 2, nx);
-    rmin = min(rmin, wrapi(i0 - 1, nx));
-    for (int q = 0; q < R + 2; ++q) rmin = min(rmin, wrapi(i0 + q, nx));
-    // integer division leaves uniform values in vector registers; the buffer
-    // offsets derived from rmin must sit in scalar registers
-    rmin = __builtin_amdgcn_readfirstlane(rmin);
-    const unsigned rbase = (unsigned)rmin * nzs + zoff;   // stored row of (rmin, z = 0)
-    const unsigned mbase = (unsigned)rmin * nz;           // moment record of (rmin, z = 0)
-    const unsigned fieldb = (unsigned)nzs * rowb;  // bytes per x-row of one field
-    // stored-row offset of interior z index jj (periodic: wrapped; open: the
-    // ghost rows hold jj in [-zoff, nz + zoff))
-    auto zrow = [&](int jj) { return (unsigned)(wrapi(jj + zoff, nzs) - zoff); };
-    const unsigned vo = (unsigned)(rw - rmin) * fieldb + (unsigned)(pl * V) * sizeof(T);
+    // The wave's x rows i0-2 .. i0+R+1 are the stored rows i0 .. i0+R+3 (the
+    // periodic ghost rows make them contiguous); rows past the last interior
+    // row + 2 map to the spare stored row nx + 4.  One descriptor per state
+    // copy covers stored rows i0 .. min(i0 + R + 4, nxs - 1); a lane's stored
+    // row relative to i0 goes into its vector offset.
+    const int spare = nx + 2 * kXg - i0;     // spare stored row, relative to i0
+    const unsigned long long woff = (unsigned long long)i0 * xrowb;
+    const unsigned wbytes = (unsigned)(min(R + 5, a.g.nxs - i0)) * xrowb;
+    const Rows<T> yin{a.yin + woff, wbytes, rowb}, uu{a.u + woff, wbytes, rowb};
+    const Rows<T> ww{a.w + woff, wbytes, rowb}, yout{a.yout + woff, wbytes, rowb};
+    // stored z row of interior z index jj (periodic: wrapped; open: the ghost
+    // rows hold jj in [-zoff, nz + zoff))
+    auto zrow = [&](int jj) { return (unsigned)wrapi(jj + zoff, nzs); };
+    const unsigned dof = (unsigned)(pl * V) * sizeof(T);
+    const unsigned vo = (unsigned)min(g + kXg, spare) * xrowb + dof;
     // Stores are unconditional (no branch, so the compiler can count the
     // outstanding memory operations across iterations exactly): a lane past
-    // the last x row writes the spare row nx of every field, a lane past nv
-    // writes its own padding columns; neither is ever read.
-    const int rst = row_ok ? rw : nx;
-    const unsigned vos = (unsigned)(rst - rmin) * fieldb + (unsigned)(p * V) * sizeof(T);
+    // the last x row writes the spare row, a lane past nv writes its own
+    // padding columns; neither is ever read by a lane that stores a result.
+    const unsigned vos = (unsigned)(row_ok ? g + kXg : spare) * xrowb + (unsigned)(p * V) * sizeof(T);
     unsigned vh[NH];
     int slot_h[NH];
 #pragma unroll
     for (int h = 0; h < NH; ++h) {
         const int k = h * R + g;   // halo index 0..3 -> rows i0-2, i0-1, i0+R, i0+R+1
-        const int hr = wrapi(k < 2 ? i0 - 2 + k : i0 + R + (k - 2), nx);
-        vh[h] = (unsigned)(hr - rmin) * fieldb + (unsigned)(pl * V) * sizeof(T);
+        const int hs = k < 2 ? k : min(R + k, spare);   // stored row relative to i0
+        vh[h] = (unsigned)hs * xrowb + dof;
         slot_h[h] = (k < 2 ? k : R + k) * D + dl * V;
     }
     const int slot_c = (g + 2) * D + dl * V;
@@ -393,7 +381,9 @@ __device__ __forceinline__ void stage_body(const StageArgs<T, FPD>& a, int lbloc
     const T mu2 = (T)a.mu2, mu = (T)a.mu, p2st = (T)a.p2st, p2ct = (T)a.p2ct;
     const T pmo = (T)a.pmo, ct2 = (T)a.ct2, st = (T)a.st;
     const T cw = (T)a.cw, cy = (T)a.cy, ca = (T)a.ca, cb = (T)a.cb;
-    const unsigned vom = (unsigned)(rw - rmin) * (unsigned)nz * (kMomStride * 8);
+    // moment records: interior sites; a lane past the last row reads row nx-1
+    const unsigned mbase = (unsigned)i0 * nz;             // record of (i0, z = 0)
+    const unsigned vom = (unsigned)(min(r, nx - 1) - i0) * (unsigned)nz * (kMomStride * 8);
 
     const int j0 = zs * a.zlen;
     const int j1 = min(nz, j0 + a.zlen);
@@ -417,20 +407,20 @@ __device__ __forceinline__ void stage_body(const StageArgs<T, FPD>& a, int lbloc
     };
 #pragma unroll
     for (int qz = 0; qz < 4; ++qz) {
-        const unsigned so = (rbase + zrow(j0 - 2 + qz)) * rowb;
+        const unsigned so = zrow(j0 - 2 + qz) * siteb;
 #pragma unroll
-        for (int f = 0; f < kNVar; ++f) a.yin.ldv(a.yin.base, f, vo, so, win[f][qz]);
+        for (int f = 0; f < kNVar; ++f) yin.ldv(yin.base, f, vo, so, win[f][qz]);
     }
     // halo rows of z = j0 and j0+1 straight into their levels
 #pragma unroll
     for (int qz = 0; qz < 2; ++qz) {
-        const unsigned so = (rbase + zrow(j0 + qz)) * rowb;
+        const unsigned so = zrow(j0 + qz) * siteb;
 #pragma unroll
         for (int h = 0; h < NH; ++h)
 #pragma unroll
             for (int f = 0; f < kNVar; ++f) {
                 T x[V];
-                a.yin.ldv(a.yin.base, f, vh[h], so, x);
+                yin.ldv(yin.base, f, vh[h], so, x);
 #pragma unroll
                 for (int e = 0; e < V; ++e) xs[(j0 + qz) & 1][f][slot_h[h] + e] = x[e];
             }
@@ -438,7 +428,7 @@ __device__ __forceinline__ void stage_body(const StageArgs<T, FPD>& a, int lbloc
     load_moments(M, j0);
 
     for (int j = j0; j < j1; ++j) {
-        const unsigned so = (rbase + j) * rowb;
+        const unsigned so = (unsigned)(j + zoff) * siteb;
         const int lv = j & 1;
         wave_sync();   // the previous iteration's LDS reads stay before these writes
         // own rows of z = j into this level (the halo rows of z = j are there)
@@ -449,25 +439,25 @@ __device__ __forceinline__ void stage_body(const StageArgs<T, FPD>& a, int lbloc
         // ---- all loads of the iteration, in the order they are consumed:
         // own row z+2 (stencil), u and w (update), halo rows z+2 (next
         // iteration), moments of z+1 (next iteration)
-        const unsigned long long by = a.yin.fresh_base();
-        const unsigned s2 = (rbase + zrow(j + 2)) * rowb;
+        const unsigned long long by = yin.fresh_base();
+        const unsigned s2 = zrow(j + 2) * siteb;
 #pragma unroll
-        for (int f = 0; f < kNVar; ++f) a.yin.ldv(by, f, vo, s2, win[f][4]);
+        for (int f = 0; f < kNVar; ++f) yin.ldv(by, f, vo, s2, win[f][4]);
         T uc[kNVar][V], wc[kNVar][V];
         if constexpr (STAGE == 2 || STAGE == 3 || STAGE == 6 || STAGE == 7) {
-            const unsigned long long bu = a.u.fresh_base();
+            const unsigned long long bu = uu.fresh_base();
 #pragma unroll
-            for (int f = 0; f < kNVar; ++f) a.u.template ldv<2>(bu, f, vo, so, uc[f]);
+            for (int f = 0; f < kNVar; ++f) uu.template ldv<2>(bu, f, vo, so, uc[f]);
         }
         if constexpr (STAGE >= 2 && STAGE <= 4) {
-            const unsigned long long bw = a.w.fresh_base();
+            const unsigned long long bw = ww.fresh_base();
 #pragma unroll
-            for (int f = 0; f < kNVar; ++f) a.w.template ldv<2>(bw, f, vo, so, wc[f]);
+            for (int f = 0; f < kNVar; ++f) ww.template ldv<2>(bw, f, vo, so, wc[f]);
         }
 #pragma unroll
         for (int h = 0; h < NH; ++h)
 #pragma unroll
-            for (int f = 0; f < kNVar; ++f) a.yin.ldv(by, f, vh[h], s2, hreg[h][f]);
+            for (int f = 0; f < kNVar; ++f) yin.ldv(by, f, vh[h], s2, hreg[h][f]);
         // unconditional (a branch here would make every later wait a full
         // drain); the last row of the box reloads itself
         load_moments(Mn, min(j + 1, nz - 1));
@@ -526,12 +516,12 @@ __device__ __forceinline__ void stage_body(const StageArgs<T, FPD>& a, int lbloc
             }
         }
         {
-            const unsigned long long bw = a.w.fresh_base();
-            const unsigned long long bo = a.yout.fresh_base();
+            const unsigned long long bw = ww.fresh_base();
+            const unsigned long long bo = yout.fresh_base();
 #pragma unroll
             for (int f = 0; f < kNVar; ++f) {
-                if constexpr (STAGE >= 1 && STAGE <= 3) a.w.stv(bw, f, wn[f], vos, so);
-                a.yout.stv(bo, f, yo[f], vos, so);
+                if constexpr (STAGE >= 1 && STAGE <= 3) ww.stv(bw, f, wn[f], vos, so);
+                yout.stv(bo, f, yo[f], vos, so);
             }
         }
         if constexpr (STAGE != 0) {
@@ -574,7 +564,7 @@ __device__ __forceinline__ void stage_body(const StageArgs<T, FPD>& a, int lbloc
             asm volatile("" : "+s"(bp));
             const __amdgpu_buffer_rsrc_t rp =
                 __builtin_amdgcn_make_buffer_rsrc((void*)bp, 0, (int)a.mpart_bytes, kRsrcWord3);
-            const unsigned rec = ((unsigned)(rst - rmin) * (unsigned)nz * nch + (unsigned)ch) * (kMomRec * 8);
+            const unsigned rec = ((unsigned)((row_ok ? r : nx) - i0) * (unsigned)nz * nch + (unsigned)ch) * (kMomRec * 8);
             const unsigned srec = (mbase + j) * (unsigned)(nch * kMomRec * 8);
             Lane<double>::st(tot, rp, rec + (unsigned)(dl >> sh) * 8, srec);
         }
@@ -604,25 +594,25 @@ __device__ __forceinline__ void stage_body(const StageArgs<T, FPD>& a, int lbloc
     }
 }
 
-template <typename T, int R, int FPD, int STAGE, bool COMP>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 2))) void k_stage(const StageArgs<T, FPD> a) {
+template <typename T, int R, int STAGE, bool COMP>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 2))) void k_stage(const StageArgs<T> a) {
     __shared__ Slab<T, R> slab[4];
     const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    stage_body<T, R, FPD, STAGE, COMP>(a, xcd_remap(blockIdx.x, gridDim.x), wib, slab[wib]);
+    stage_body<T, R, STAGE, COMP>(a, xcd_remap(blockIdx.x, gridDim.x), wib, slab[wib]);
 }
 
 // Many independent runs in one launch: logical block -> (run, block of run)
 // from bmap; each run's arguments from the runs table (read with scalar loads).
-template <typename T, int R, int FPD, int STAGE, bool COMP>
+template <typename T, int R, int STAGE, bool COMP>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 2))) void k_stage_batch(
-    const StageArgs<T, FPD>* __restrict__ runs, const int* __restrict__ bmap) {
+    const StageArgs<T>* __restrict__ runs, const int* __restrict__ bmap) {
     __shared__ Slab<T, R> slab[4];
     const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lb = xcd_remap(blockIdx.x, gridDim.x);
     const __attribute__((address_space(4))) int* cm = (const __attribute__((address_space(4))) int*)bmap;
     const int run = cm[2 * lb], blk = cm[2 * lb + 1];
-    const StageArgs<T, FPD> a = runs[run];   // uniform, read-only: scalar loads
-    stage_body<T, R, FPD, STAGE, COMP>(a, blk, wib, slab[wib]);
+    const StageArgs<T> a = runs[run];   // uniform, read-only: scalar loads
+    stage_body<T, R, STAGE, COMP>(a, blk, wib, slab[wib]);
 }
 
 int rows_per_wave(int esize) { return esize == 8 ? StageShape<double>::R : StageShape<float>::R; }
@@ -663,37 +653,24 @@ StagePlan plan_stage_zlen(const Geometry& g, int zlen) {
 }
 
 bool buffer_addressable(const Geometry& g, int esize) {
-    return (long long)g.fs * esize < (1ll << 32) - 1;
+    const long long lim = (1ll << 32) - 1;
+    const int R = rows_per_wave(esize);
+    const long long wave_rows = std::min<long long>(R + 5, g.nxs) * g.xrow * esize;
+    const long long part = (long long)(g.nx + 1) * g.nz * g.nch * kMomRec * (long long)sizeof(double);
+    const long long tot = (long long)g.nx * g.nz * kMomStride * (long long)sizeof(double);
+    return wave_rows < lim && part < lim && tot < lim;
 }
 
-// fields per buffer descriptor: the largest of 4, 2, 1 whose group spans
-// fewer than 4 GiB
-static int fields_per_descriptor(const Geometry& g, int esize) {
-    const long long fb = g.fs * esize;
-    for (int fpd : {4, 2}) if (fb * fpd < (1ll << 32) - 1) return fpd;
-    return 1;
-}
-
-template <typename T, int FPD>
-static Fields<T, FPD> fields_of(const Geometry& g, const T* p) {
-    Fields<T, FPD> f;
-    f.base = (unsigned long long)p;
-    f.field_bytes = (unsigned)(g.fs * (long long)sizeof(T));
-    f.group_bytes = (unsigned)(g.fs * (long long)sizeof(T) * FPD);
-    f.group_stride = (unsigned long long)f.group_bytes;
-    return f;
-}
-
-template <typename T, int FPD>
-static StageArgs<T, FPD> make_stage_args(const Geometry& g, const Physics& ph, const StagePlan& pl, const T* yin,
-                                         const T* u, T* w, T* yout, const double* mtot, double* mpart, double cw,
-                                         double cy, double ca = 0.0, double cb = 0.0) {
-    StageArgs<T, FPD> a;
+template <typename T>
+static StageArgs<T> make_stage_args(const Geometry& g, const Physics& ph, const StagePlan& pl, const T* yin,
+                                    const T* u, T* w, T* yout, const double* mtot, double* mpart, double cw,
+                                    double cy, double ca = 0.0, double cb = 0.0) {
+    StageArgs<T> a;
     a.g = g;
-    a.yin = fields_of<T, FPD>(g, yin);
-    a.u = fields_of<T, FPD>(g, u ? u : yin);
-    a.w = fields_of<T, FPD>(g, w ? w : yout);
-    a.yout = fields_of<T, FPD>(g, yout);
+    a.yin = (unsigned long long)yin;
+    a.u = (unsigned long long)(u ? u : yin);
+    a.w = (unsigned long long)(w ? w : yout);
+    a.yout = (unsigned long long)yout;
     a.mtot = (unsigned long long)mtot;
     a.mtot_bytes = (unsigned)((long long)g.nx * g.nz * kMomStride * sizeof(double));
     a.mpart = (unsigned long long)mpart;
@@ -714,39 +691,74 @@ static StageArgs<T, FPD> make_stage_args(const Geometry& g, const Physics& ph, c
     return a;
 }
 
-template <typename T, int FPD>
-static hipError_t launch_stage_fpd(int stage, const Geometry& g, const Physics& ph, const StagePlan& pl,
-                                   const T* yin, const T* u, T* w, T* yout, const double* mtot, double* mpart,
-                                   double cw, double cy, hipStream_t s, double ca, double cb) {
-    const StageArgs<T, FPD> a = make_stage_args<T, FPD>(g, ph, pl, yin, u, w, yout, mtot, mpart, cw, cy, ca, cb);
-    constexpr bool COMP = sizeof(T) == 4;
-    constexpr int R = StageShape<T>::R;
-    if (g.lpr != dirs_per_chunk((int)sizeof(T))) return hipErrorInvalidValue;
-    dim3 grid(pl.nblocks), block(256);
-    switch (stage) {
-        case 0: hipLaunchKernelGGL((k_stage<T, R, FPD, 0, COMP>), grid, block, 0, s, a); break;
-        case 1: hipLaunchKernelGGL((k_stage<T, R, FPD, 1, COMP>), grid, block, 0, s, a); break;
-        case 2: hipLaunchKernelGGL((k_stage<T, R, FPD, 2, COMP>), grid, block, 0, s, a); break;
-        case 3: hipLaunchKernelGGL((k_stage<T, R, FPD, 3, COMP>), grid, block, 0, s, a); break;
-        case 4: hipLaunchKernelGGL((k_stage<T, R, FPD, 4, COMP>), grid, block, 0, s, a); break;
-        case 5: hipLaunchKernelGGL((k_stage<T, R, FPD, 5, COMP>), grid, block, 0, s, a); break;
-        case 6: hipLaunchKernelGGL((k_stage<T, R, FPD, 6, COMP>), grid, block, 0, s, a); break;
-        case 7: hipLaunchKernelGGL((k_stage<T, R, FPD, 7, COMP>), grid, block, 0, s, a); break;
-        default: return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
-}
-
 template <typename T>
 hipError_t launch_stage(int stage, const Geometry& g, const Physics& ph, const StagePlan& pl,
                         const T* yin, const T* u, T* w, T* yout, const double* mtot, double* mpart,
                         double cw, double cy, hipStream_t s, double ca, double cb) {
     if (!buffer_addressable(g, sizeof(T))) return hipErrorInvalidValue;
-    switch (fields_per_descriptor(g, sizeof(T))) {
-        case 4: return launch_stage_fpd<T, 4>(stage, g, ph, pl, yin, u, w, yout, mtot, mpart, cw, cy, s, ca, cb);
-        case 2: return launch_stage_fpd<T, 2>(stage, g, ph, pl, yin, u, w, yout, mtot, mpart, cw, cy, s, ca, cb);
-        default: return launch_stage_fpd<T, 1>(stage, g, ph, pl, yin, u, w, yout, mtot, mpart, cw, cy, s, ca, cb);
+    const StageArgs<T> a = make_stage_args<T>(g, ph, pl, yin, u, w, yout, mtot, mpart, cw, cy, ca, cb);
+    constexpr bool COMP = sizeof(T) == 4;
+    constexpr int R = StageShape<T>::R;
+    if (g.lpr != dirs_per_chunk((int)sizeof(T))) return hipErrorInvalidValue;
+    dim3 grid(pl.nblocks), block(256);
+    switch (stage) {
+        case 0: hipLaunchKernelGGL((k_stage<T, R, 0, COMP>), grid, block, 0, s, a); break;
+        case 1: hipLaunchKernelGGL((k_stage<T, R, 1, COMP>), grid, block, 0, s, a); break;
+        case 2: hipLaunchKernelGGL((k_stage<T, R, 2, COMP>), grid, block, 0, s, a); break;
+        case 3: hipLaunchKernelGGL((k_stage<T, R, 3, COMP>), grid, block, 0, s, a); break;
+        case 4: hipLaunchKernelGGL((k_stage<T, R, 4, COMP>), grid, block, 0, s, a); break;
+        case 5: hipLaunchKernelGGL((k_stage<T, R, 5, COMP>), grid, block, 0, s, a); break;
+        case 6: hipLaunchKernelGGL((k_stage<T, R, 6, COMP>), grid, block, 0, s, a); break;
+        case 7: hipLaunchKernelGGL((k_stage<T, R, 7, COMP>), grid, block, 0, s, a); break;
+        default: return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// periodic x ghost rows of a stage input: stored rows 0, 1, nx+2, nx+3 are
+// copies of the stored rows of interior x rows nx-2, nx-1, 0, 1 (whole stored
+// rows, z ghost rows included), moved as 16-byte words
+// ---------------------------------------------------------------------------
+struct XgArgs {
+    unsigned long long y;   // state copy
+    long long words;        // 16-byte words per stored x row
+    int nx;
+};
+constexpr int kXgPerThread = 4;
+
+long long xg_words(const Geometry& g, int esize) { return g.xrow * esize / 16; }
+long long xg_blocks(long long words) { return 4 * ((words + 256 * kXgPerThread - 1) / (256 * kXgPerThread)); }
+
+__device__ __forceinline__ void xg_body(const XgArgs& a, long long blk) {
+    const long long per = (a.words + 256 * kXgPerThread - 1) / (256 * kXgPerThread);
+    const int k = (int)(blk / per);                 // ghost row 0..3
+    const long long c = blk % per;
+    const int dst = k < 2 ? k : a.nx + k;           // stored rows 0, 1, nx+2, nx+3
+    const int src = wrapi(dst - kXg, a.nx) + kXg;   // stored row of its interior x row
+    uint4* d = reinterpret_cast<uint4*>(a.y) + dst * a.words;
+    const uint4* sp = reinterpret_cast<const uint4*>(a.y) + src * a.words;
+#pragma unroll
+    for (int q = 0; q < kXgPerThread; ++q) {
+        const long long n = (c * kXgPerThread + q) * 256 + threadIdx.x;
+        if (n < a.words) d[n] = sp[n];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_xghosts(const XgArgs a) { xg_body(a, blockIdx.x); }
+
+__global__ __launch_bounds__(256) void k_xghosts_batch(const XgArgs* __restrict__ runs, const int* __restrict__ xmap) {
+    const __attribute__((address_space(4))) int* cm = (const __attribute__((address_space(4))) int*)xmap;
+    const int run = cm[2 * blockIdx.x], blk = cm[2 * blockIdx.x + 1];
+    const XgArgs a = runs[run];
+    xg_body(a, blk);
+}
+
+template <typename T>
+hipError_t launch_xghosts(const Geometry& g, T* y, hipStream_t s) {
+    const XgArgs a{(unsigned long long)y, xg_words(g, sizeof(T)), g.nx};
+    hipLaunchKernelGGL(k_xghosts, dim3((unsigned)xg_blocks(a.words)), dim3(256), 0, s, a);
+    return hipGetLastError();
 }
 
 struct TotArgs {
@@ -812,10 +824,10 @@ __global__ __launch_bounds__(256) void k_moment_totals_batch(const TotArgs* __re
     totals_body<COMP>(ta.part, ta.nch, ta.nsites, ta.tot, (long long)blk * 256 + threadIdx.x);
 }
 
-template <typename T, int FPD>
+template <typename T>
 static hipError_t batch_build_t(BatchTables& t, const std::vector<BatchMember>& ms, hipStream_t s) {
     const int n = (int)ms.size();
-    std::vector<int> bmap, tmap;
+    std::vector<int> bmap, tmap, xmap;
     for (int r = 0; r < n; ++r)
         for (int b = 0; b < ms[r].pl.nblocks; ++b) {
             bmap.push_back(r);
@@ -829,23 +841,31 @@ static hipError_t batch_build_t(BatchTables& t, const std::vector<BatchMember>& 
             tmap.push_back(b);
         }
     }
+    for (int r = 0; r < n; ++r) {
+        const long long nb = xg_blocks(xg_words(ms[r].g, sizeof(T)));
+        for (long long b = 0; b < nb; ++b) {
+            xmap.push_back(r);
+            xmap.push_back((int)b);
+        }
+    }
     t.nblocks = (int)bmap.size() / 2;
     t.tblocks = (int)tmap.size() / 2;
+    t.xblocks = (int)xmap.size() / 2;
     hipError_t e;
-    if ((e = hipMalloc((void**)&t.bmap, bmap.size() * sizeof(int))) != hipSuccess) return e;
-    if ((e = hipMalloc((void**)&t.tmap, tmap.size() * sizeof(int))) != hipSuccess) return e;
-    if ((e = hipMemcpyAsync(t.bmap, bmap.data(), bmap.size() * sizeof(int), hipMemcpyHostToDevice, s)) != hipSuccess)
-        return e;
-    if ((e = hipMemcpyAsync(t.tmap, tmap.data(), tmap.size() * sizeof(int), hipMemcpyHostToDevice, s)) != hipSuccess)
-        return e;
+    for (auto [dst, v] : {std::pair<int**, std::vector<int>*>{&t.bmap, &bmap}, {&t.tmap, &tmap}, {&t.xmap, &xmap}}) {
+        if ((e = hipMalloc((void**)dst, v->size() * sizeof(int))) != hipSuccess) return e;
+        if ((e = hipMemcpyAsync(*dst, v->data(), v->size() * sizeof(int), hipMemcpyHostToDevice, s)) != hipSuccess)
+            return e;
+    }
     for (int st = 1; st <= 4; ++st) {
-        std::vector<StageArgs<T, FPD>> args(n);
+        std::vector<StageArgs<T>> args(n);
         std::vector<TotArgs> tots(n);
+        std::vector<XgArgs> xgs(n);
         for (int r = 0; r < n; ++r) {
             const BatchMember& m = ms[r];
             const double dt = m.dt, c6 = 1.0 / 6.0 * dt, c3 = 1.0 / 3.0 * dt, c2 = 0.5 * dt;
             T *u = (T*)m.u, *w = (T*)m.w, *ya = (T*)m.ya, *yb = (T*)m.yb;
-            const T* yin;
+            T* yin;
             const T* uu;
             T* yout;
             const double* mt;
@@ -857,15 +877,20 @@ static hipError_t batch_build_t(BatchTables& t, const std::vector<BatchMember>& 
                 case 3: yin = yb; uu = u; yout = ya; mt = m.mb; mo = m.ma; cw = c3; cy = dt; break;
                 default: yin = ya; uu = ya; yout = u; mt = m.ma; mo = m.mu; cw = c6; cy = 0.0; break;
             }
-            args[r] = make_stage_args<T, FPD>(m.g, m.ph, m.pl, yin, uu, w, yout, mt, m.mpart, cw, cy);
+            args[r] = make_stage_args<T>(m.g, m.ph, m.pl, yin, uu, w, yout, mt, m.mpart, cw, cy);
             tots[r] = TotArgs{m.mpart, mo, m.g.nch, (long long)m.g.nx * m.g.nz};
+            xgs[r] = XgArgs{(unsigned long long)yin, xg_words(m.g, sizeof(T)), m.g.nx};
         }
-        if ((e = hipMalloc(&t.args[st - 1], n * sizeof(StageArgs<T, FPD>))) != hipSuccess) return e;
+        if ((e = hipMalloc(&t.args[st - 1], n * sizeof(StageArgs<T>))) != hipSuccess) return e;
         if ((e = hipMalloc(&t.tot[st - 1], n * sizeof(TotArgs))) != hipSuccess) return e;
-        if ((e = hipMemcpyAsync(t.args[st - 1], args.data(), n * sizeof(StageArgs<T, FPD>), hipMemcpyHostToDevice,
-                                s)) != hipSuccess)
+        if ((e = hipMalloc(&t.xg[st - 1], n * sizeof(XgArgs))) != hipSuccess) return e;
+        if ((e = hipMemcpyAsync(t.args[st - 1], args.data(), n * sizeof(StageArgs<T>), hipMemcpyHostToDevice, s)) !=
+            hipSuccess)
             return e;
         if ((e = hipMemcpyAsync(t.tot[st - 1], tots.data(), n * sizeof(TotArgs), hipMemcpyHostToDevice, s)) !=
+            hipSuccess)
+            return e;
+        if ((e = hipMemcpyAsync(t.xg[st - 1], xgs.data(), n * sizeof(XgArgs), hipMemcpyHostToDevice, s)) !=
             hipSuccess)
             return e;
     }
@@ -876,64 +901,38 @@ hipError_t batch_build(BatchTables& t, const std::vector<BatchMember>& ms, int e
     batch_free(t);
     t.esize = esize;
     if (ms.empty()) return hipSuccess;
-    int fpd = 4;
     for (const auto& m : ms) {
         if (!buffer_addressable(m.g, esize)) return hipErrorInvalidValue;
-        fpd = std::min(fpd, fields_per_descriptor(m.g, esize));
         if (m.g.lpr != dirs_per_chunk(esize)) return hipErrorInvalidValue;
     }
-    t.fpd = fpd;
-    if (esize == 8) {
-        if (fpd == 4) return batch_build_t<double, 4>(t, ms, s);
-        if (fpd == 2) return batch_build_t<double, 2>(t, ms, s);
-        return batch_build_t<double, 1>(t, ms, s);
-    }
-    if (fpd == 4) return batch_build_t<float, 4>(t, ms, s);
-    if (fpd == 2) return batch_build_t<float, 2>(t, ms, s);
-    return batch_build_t<float, 1>(t, ms, s);
+    return esize == 8 ? batch_build_t<double>(t, ms, s) : batch_build_t<float>(t, ms, s);
 }
 
 void batch_free(BatchTables& t) {
     for (int k = 0; k < 4; ++k) {
         if (t.args[k]) (void)hipFree(t.args[k]);
         if (t.tot[k]) (void)hipFree(t.tot[k]);
-        t.args[k] = t.tot[k] = nullptr;
+        if (t.xg[k]) (void)hipFree(t.xg[k]);
+        t.args[k] = t.tot[k] = t.xg[k] = nullptr;
     }
-    if (t.bmap) (void)hipFree(t.bmap);
-    if (t.tmap) (void)hipFree(t.tmap);
-    t.bmap = t.tmap = nullptr;
-    t.nblocks = t.tblocks = 0;
+    for (int** m : {&t.bmap, &t.tmap, &t.xmap}) {
+        if (*m) (void)hipFree(*m);
+        *m = nullptr;
+    }
+    t.nblocks = t.tblocks = t.xblocks = 0;
 }
 
-template <typename T, int FPD>
-static hipError_t batch_step_t(const BatchTables& t, hipStream_t s) {
-    constexpr int R = StageShape<T>::R;
-    constexpr bool COMP = sizeof(T) == 4;
-    const dim3 grid(t.nblocks), block(256), tgrid(t.tblocks);
-    for (int st = 1; st <= 4; ++st) {
-        const StageArgs<T, FPD>* a = (const StageArgs<T, FPD>*)t.args[st - 1];
-        switch (st) {
-            case 1: hipLaunchKernelGGL((k_stage_batch<T, R, FPD, 1, COMP>), grid, block, 0, s, a, t.bmap); break;
-            case 2: hipLaunchKernelGGL((k_stage_batch<T, R, FPD, 2, COMP>), grid, block, 0, s, a, t.bmap); break;
-            case 3: hipLaunchKernelGGL((k_stage_batch<T, R, FPD, 3, COMP>), grid, block, 0, s, a, t.bmap); break;
-            default: hipLaunchKernelGGL((k_stage_batch<T, R, FPD, 4, COMP>), grid, block, 0, s, a, t.bmap); break;
-        }
-        hipLaunchKernelGGL((k_moment_totals_batch<COMP>), tgrid, block, 0, s, (const TotArgs*)t.tot[st - 1], t.tmap);
-    }
-    return hipGetLastError();
-}
-
-template <typename T, int FPD>
+template <typename T>
 static hipError_t batch_stage_t(const BatchTables& t, int st, hipStream_t s) {
     constexpr int R = StageShape<T>::R;
     constexpr bool COMP = sizeof(T) == 4;
     const dim3 grid(t.nblocks), block(256);
-    const StageArgs<T, FPD>* a = (const StageArgs<T, FPD>*)t.args[st - 1];
+    const StageArgs<T>* a = (const StageArgs<T>*)t.args[st - 1];
     switch (st) {
-        case 1: hipLaunchKernelGGL((k_stage_batch<T, R, FPD, 1, COMP>), grid, block, 0, s, a, t.bmap); break;
-        case 2: hipLaunchKernelGGL((k_stage_batch<T, R, FPD, 2, COMP>), grid, block, 0, s, a, t.bmap); break;
-        case 3: hipLaunchKernelGGL((k_stage_batch<T, R, FPD, 3, COMP>), grid, block, 0, s, a, t.bmap); break;
-        default: hipLaunchKernelGGL((k_stage_batch<T, R, FPD, 4, COMP>), grid, block, 0, s, a, t.bmap); break;
+        case 1: hipLaunchKernelGGL((k_stage_batch<T, R, 1, COMP>), grid, block, 0, s, a, t.bmap); break;
+        case 2: hipLaunchKernelGGL((k_stage_batch<T, R, 2, COMP>), grid, block, 0, s, a, t.bmap); break;
+        case 3: hipLaunchKernelGGL((k_stage_batch<T, R, 3, COMP>), grid, block, 0, s, a, t.bmap); break;
+        default: hipLaunchKernelGGL((k_stage_batch<T, R, 4, COMP>), grid, block, 0, s, a, t.bmap); break;
     }
     return hipGetLastError();
 }
@@ -941,26 +940,26 @@ static hipError_t batch_stage_t(const BatchTables& t, int st, hipStream_t s) {
 hipError_t batch_stage(const BatchTables& t, int stage, hipStream_t s) {
     if (t.nblocks == 0) return hipSuccess;
     if (stage < 1 || stage > 4) return hipErrorInvalidValue;
-    if (t.esize == 8) {
-        if (t.fpd == 4) return batch_stage_t<double, 4>(t, stage, s);
-        if (t.fpd == 2) return batch_stage_t<double, 2>(t, stage, s);
-        return batch_stage_t<double, 1>(t, stage, s);
-    }
-    if (t.fpd == 4) return batch_stage_t<float, 4>(t, stage, s);
-    if (t.fpd == 2) return batch_stage_t<float, 2>(t, stage, s);
-    return batch_stage_t<float, 1>(t, stage, s);
+    return t.esize == 8 ? batch_stage_t<double>(t, stage, s) : batch_stage_t<float>(t, stage, s);
 }
 
+// per stage: x ghost rows of every member's stage input, the stage kernel,
+// the moment totals of its output
 hipError_t batch_step(const BatchTables& t, hipStream_t s) {
     if (t.nblocks == 0) return hipSuccess;
-    if (t.esize == 8) {
-        if (t.fpd == 4) return batch_step_t<double, 4>(t, s);
-        if (t.fpd == 2) return batch_step_t<double, 2>(t, s);
-        return batch_step_t<double, 1>(t, s);
+    const bool comp = t.esize == 4;
+    for (int st = 1; st <= 4; ++st) {
+        hipLaunchKernelGGL(k_xghosts_batch, dim3(t.xblocks), dim3(256), 0, s, (const XgArgs*)t.xg[st - 1], t.xmap);
+        const hipError_t e = t.esize == 8 ? batch_stage_t<double>(t, st, s) : batch_stage_t<float>(t, st, s);
+        if (e != hipSuccess) return e;
+        if (comp)
+            hipLaunchKernelGGL((k_moment_totals_batch<true>), dim3(t.tblocks), dim3(256), 0, s,
+                               (const TotArgs*)t.tot[st - 1], t.tmap);
+        else
+            hipLaunchKernelGGL((k_moment_totals_batch<false>), dim3(t.tblocks), dim3(256), 0, s,
+                               (const TotArgs*)t.tot[st - 1], t.tmap);
     }
-    if (t.fpd == 4) return batch_step_t<float, 4>(t, s);
-    if (t.fpd == 2) return batch_step_t<float, 2>(t, s);
-    return batch_step_t<float, 1>(t, s);
+    return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
@@ -979,11 +978,11 @@ __global__ __launch_bounds__(256) void k_moments(const Geometry g, const T* __re
 #pragma unroll
     for (int s = 0; s < kNMom; ++s) { S[s] = 0.0; E[s] = 0.0; }
     const int v1 = min(g.nv, (ch + 1) * g.lpr);
-    const long long so = sto(g, site) * g.nvp;
+    const long long so = site_off(g, site);
     for (int v = ch * g.lpr; v < v1; ++v) {
         T yv[kNVar];
 #pragma unroll
-        for (int f = 0; f < kNVar; ++f) yv[f] = y[f * g.fs + so + v];
+        for (int f = 0; f < kNVar; ++f) yv[f] = y[so + f * g.nvp + v];
         const double vw = g.vw[v];
         const double q[4] = {vw, g.vx[v] * vw, g.vy[v] * vw, g.vz[v] * vw};
         double t[16];
@@ -1034,10 +1033,10 @@ __global__ __launch_bounds__(256) void k_init(const Geometry g, int ipt, const d
     if (n >= total) return;
     const long long site = n / g.nvp;
     const int v = (int)(n % g.nvp);
-    const long long o = sto(g, site) * g.nvp + v;
+    const long long o = site_off(g, site) + v;
     if (v >= g.nv) {  // row padding
 #pragma unroll
-        for (int f = 0; f < kNVar; ++f) y[f * g.fs + o] = (T)0;
+        for (int f = 0; f < kNVar; ++f) y[o + f * g.nvp] = (T)0;
         return;
     }
     const int j = (int)(site % g.nz);
@@ -1057,14 +1056,14 @@ __global__ __launch_bounds__(256) void k_init(const Geometry g, int ipt, const d
         p3o = p3o_z[j];
     }
     const double hg = 0.5 * G0[v], hgb = 0.5 * G0b[v];
-    y[0 * g.fs + o] = (T)(hg * (1.0 + p3o));
-    y[1 * g.fs + o] = (T)(hg * (1.0 - p3o));
-    y[2 * g.fs + o] = (T)(hg * tmpr);
-    y[3 * g.fs + o] = (T)(ipt == 5 ? hg * tmpi : 0.0);
-    y[4 * g.fs + o] = (T)(hgb * (1.0 + p3o));
-    y[5 * g.fs + o] = (T)(hgb * (1.0 - p3o));
-    y[6 * g.fs + o] = (T)(hgb * tmpr);
-    y[7 * g.fs + o] = (T)(ipt == 5 ? -hgb * tmpi : 0.0);
+    y[o + 0 * g.nvp] = (T)(hg * (1.0 + p3o));
+    y[o + 1 * g.nvp] = (T)(hg * (1.0 - p3o));
+    y[o + 2 * g.nvp] = (T)(hg * tmpr);
+    y[o + 3 * g.nvp] = (T)(ipt == 5 ? hg * tmpi : 0.0);
+    y[o + 4 * g.nvp] = (T)(hgb * (1.0 + p3o));
+    y[o + 5 * g.nvp] = (T)(hgb * (1.0 - p3o));
+    y[o + 6 * g.nvp] = (T)(hgb * tmpr);
+    y[o + 7 * g.nvp] = (T)(ipt == 5 ? -hgb * tmpi : 0.0);
 }
 
 template <typename T>
@@ -1105,12 +1104,12 @@ __global__ __launch_bounds__(kAnaThreads) void k_analysis(const Geometry g, cons
     for (long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x; n < pts; n += stride) {
         const long long site = n / g.nv;
         const int v = (int)(n % g.nv);
-        const long long o = sto(g, site) * g.nvp + v;
+        const long long o = site_off(g, site) + v;
         double f[kNVar];
         bool finite = true;
 #pragma unroll
         for (int q = 0; q < kNVar; ++q) {
-            f[q] = (double)y[q * g.fs + o];
+            f[q] = (double)y[o + q * g.nvp];
             finite = finite && __builtin_isfinite(f[q]);
         }
         const double w = g.vw[v], g0 = G0[v], g0b = G0b[v];
@@ -1209,7 +1208,7 @@ hipError_t launch_analysis(const Geometry& g, const T* y, const double* G0, cons
 // layout conversions, snapshots, renormalisation
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ void k_ghosted(const Geometry g, int gx, int gz, const T* __restrict__ fld,
+__global__ void k_ghosted(const Geometry g, int gx, int gz, const T* __restrict__ y, int f,
                           double* __restrict__ dst) {
     const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long nzg = g.nz + 2 * gz;
@@ -1223,48 +1222,48 @@ __global__ void k_ghosted(const Geometry g, int gx, int gz, const T* __restrict_
     if (!xin && !zin) { dst[n] = 0.0; return; }  // corners are never filled
     // x periodic by wrap; z periodic by wrap, or (open z) the stored ghost rows
     const int iw = wrapi(i, g.nx), jw = g.zoff ? j : wrapi(j, g.nz);
-    dst[n] = (double)fld[((long long)iw * g.nzs + jw + g.zoff) * g.nvp + v];
+    dst[n] = (double)y[elem_off(g, iw, jw + g.zoff, f) + v];
 }
 
 template <typename T>
-hipError_t launch_ghosted(const Geometry& g, int gx, int gz, const T* field, double* dst, hipStream_t s) {
+hipError_t launch_ghosted(const Geometry& g, int gx, int gz, const T* y, int f, double* dst, hipStream_t s) {
     const long long total = (long long)(g.nx + 2 * gx) * (g.nz + 2 * gz) * g.nv;
     hipLaunchKernelGGL((k_ghosted<T>), dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, g, gx, gz,
-                       field, dst);
+                       y, f, dst);
     return hipGetLastError();
 }
 
 template <typename T>
-__global__ void k_pack(const Geometry g, const T* __restrict__ fld, double* __restrict__ dst) {
+__global__ void k_pack(const Geometry g, const T* __restrict__ y, int f, double* __restrict__ dst) {
     const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long total = (long long)g.nx * g.nz * g.nv;
     if (n >= total) return;
     const long long s = n / g.nv;
     const int v = (int)(n % g.nv);
-    dst[n] = (double)fld[sto(g, s) * g.nvp + v];
+    dst[n] = (double)y[site_off(g, s) + f * g.nvp + v];
 }
 
 template <typename T>
-__global__ void k_unpack(const Geometry g, const double* __restrict__ src, T* __restrict__ fld) {
+__global__ void k_unpack(const Geometry g, const double* __restrict__ src, T* __restrict__ y, int f) {
     const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long total = (long long)g.nx * g.nz * g.nvp;
     if (n >= total) return;
     const long long s = n / g.nvp;
     const int v = (int)(n % g.nvp);
-    fld[sto(g, s) * g.nvp + v] = (v < g.nv) ? (T)src[s * g.nv + v] : (T)0;
+    y[site_off(g, s) + f * g.nvp + v] = (v < g.nv) ? (T)src[s * g.nv + v] : (T)0;
 }
 
 template <typename T>
-hipError_t launch_pack(const Geometry& g, const T* field, double* dst, hipStream_t s) {
+hipError_t launch_pack(const Geometry& g, const T* y, int f, double* dst, hipStream_t s) {
     const long long total = (long long)g.nx * g.nz * g.nv;
-    hipLaunchKernelGGL((k_pack<T>), dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, g, field, dst);
+    hipLaunchKernelGGL((k_pack<T>), dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, g, y, f, dst);
     return hipGetLastError();
 }
 
 template <typename T>
-hipError_t launch_unpack(const Geometry& g, const double* src, T* field, hipStream_t s) {
+hipError_t launch_unpack(const Geometry& g, const double* src, T* y, int f, hipStream_t s) {
     const long long total = (long long)g.nx * g.nz * g.nvp;
-    hipLaunchKernelGGL((k_unpack<T>), dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, g, src, field);
+    hipLaunchKernelGGL((k_unpack<T>), dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, g, src, y, f);
     return hipGetLastError();
 }
 
@@ -1279,9 +1278,9 @@ __global__ void k_p3_slice(const Geometry g, const T* __restrict__ y, const doub
     const long long r = n / sv;
     const int j = (int)(r % g.nz), a = (int)(r / g.nz);
     const int v = vi[b];
-    const long long o = ((long long)xi[a] * g.nzs + j + g.zoff) * g.nvp + v;
+    const long long o = elem_off(g, xi[a], j + g.zoff, 0) + v;
     const double iG = 1.0 / G0[v];  // nuosc_ana.cpp:10,14
-    dst[n] = ((double)y[0 * g.fs + o] - (double)y[1 * g.fs + o]) * iG;
+    dst[n] = ((double)y[o + 0 * g.nvp] - (double)y[o + 1 * g.nvp]) * iG;
 }
 
 template <typename T>
@@ -1303,24 +1302,24 @@ __global__ void k_renorm(const Geometry g, T* __restrict__ y, const double* __re
     if (n >= total) return;
     const int v = (int)(n % g.nvp);
     if (v >= g.nv) return;
-    const long long o = sto(g, n / g.nvp) * g.nvp + v;
+    const long long o = site_off(g, n / g.nvp) + v;
     double f[kNVar];
 #pragma unroll
-    for (int q = 0; q < kNVar; ++q) f[q] = (double)y[q * g.fs + o];
+    for (int q = 0; q < kNVar; ++q) f[q] = (double)y[o + q * g.nvp];
     const double iG = 1.0 / G0[v], iGb = 1.0 / G0b[v];
     const double P1 = f[2] * iG, P2 = -f[3] * iG, P3 = (f[0] - f[1]) * iG;
     const double P1b = f[6] * iGb, P2b = f[7] * iGb, P3b = (f[4] - f[5]) * iGb;
     const double iP = 1.0 / sqrt(P1 * P1 + P2 * P2 + P3 * P3);
     const double iPb = 1.0 / sqrt(P1b * P1b + P2b * P2b + P3b * P3b);
     const double tmp = iP * P3 * G0[v], tmpb = iPb * P3b * G0b[v];
-    y[0 * g.fs + o] = (T)(G0[v] + tmp);
-    y[1 * g.fs + o] = (T)(G0[v] - tmp);
-    y[2 * g.fs + o] = (T)(f[2] * iP);
-    y[3 * g.fs + o] = (T)(f[3] * iP);
-    y[4 * g.fs + o] = (T)(G0b[v] + tmpb);
-    y[5 * g.fs + o] = (T)(G0b[v] - tmpb);
-    y[6 * g.fs + o] = (T)(f[6] * iPb);
-    y[7 * g.fs + o] = (T)(f[7] * iPb);
+    y[o + 0 * g.nvp] = (T)(G0[v] + tmp);
+    y[o + 1 * g.nvp] = (T)(G0[v] - tmp);
+    y[o + 2 * g.nvp] = (T)(f[2] * iP);
+    y[o + 3 * g.nvp] = (T)(f[3] * iP);
+    y[o + 4 * g.nvp] = (T)(G0b[v] + tmpb);
+    y[o + 5 * g.nvp] = (T)(G0b[v] - tmpb);
+    y[o + 6 * g.nvp] = (T)(f[6] * iPb);
+    y[o + 7 * g.nvp] = (T)(f[7] * iPb);
 }
 
 template <typename T>
@@ -1351,7 +1350,7 @@ __global__ __launch_bounds__(256) void k_open_ghosts(const Geometry g, T* __rest
     const int side = (int)(r % 2);
     const int i = (int)(r / 2);
     const int j = side == 0 ? -q : g.nz - 1 + q;
-    const long long o = ((long long)i * g.nzs + j + g.zoff) * g.nvp + v;
+    const long long o = elem_off(g, i, j + g.zoff, 0) + v;
     const double vz = g.vz[v];
     const bool inflow = side == 0 ? (vz > 0.0) : (vz < 0.0);
     if (inflow) {
@@ -1364,26 +1363,26 @@ __global__ __launch_bounds__(256) void k_open_ghosts(const Geometry g, T* __rest
         }
         const double p3o = sqrt(1.0 - a * a);
         const double hg = 0.5 * G0[v], hgb = 0.5 * G0b[v];
-        y[0 * g.fs + o] = (T)(hg * (1.0 + p3o));
-        y[1 * g.fs + o] = (T)(hg * (1.0 - p3o));
-        y[2 * g.fs + o] = (T)(hg * a);
-        y[3 * g.fs + o] = (T)0.0;
-        y[4 * g.fs + o] = (T)(hgb * (1.0 + p3o));
-        y[5 * g.fs + o] = (T)(hgb * (1.0 - p3o));
-        y[6 * g.fs + o] = (T)(hgb * a);
-        y[7 * g.fs + o] = (T)0.0;
+        y[o + 0 * g.nvp] = (T)(hg * (1.0 + p3o));
+        y[o + 1 * g.nvp] = (T)(hg * (1.0 - p3o));
+        y[o + 2 * g.nvp] = (T)(hg * a);
+        y[o + 3 * g.nvp] = (T)0.0;
+        y[o + 4 * g.nvp] = (T)(hgb * (1.0 + p3o));
+        y[o + 5 * g.nvp] = (T)(hgb * (1.0 - p3o));
+        y[o + 6 * g.nvp] = (T)(hgb * a);
+        y[o + 7 * g.nvp] = (T)0.0;
     } else {
         // quadratic extrapolation through the three edge cells
         const double w0 = q == 1 ? 3.0 : (q == 2 ? 6.0 : 10.0);
         const double w1 = q == 1 ? -3.0 : (q == 2 ? -8.0 : -15.0);
         const double w2 = q == 1 ? 1.0 : (q == 2 ? 3.0 : 6.0);
         const int e0 = side == 0 ? 0 : g.nz - 1, dir = side == 0 ? 1 : -1;
-        const long long c0 = ((long long)i * g.nzs + e0 + g.zoff) * g.nvp + v;
-        const long long step = (long long)dir * g.nvp;
+        const long long c0 = elem_off(g, i, e0 + g.zoff, 0) + v;
+        const long long step = (long long)dir * kNVar * g.nvp;   // one site record along z
 #pragma unroll
         for (int f = 0; f < kNVar; ++f) {
-            const T* p = y + f * g.fs + c0;
-            y[f * g.fs + o] = (T)(w0 * (double)p[0] + w1 * (double)p[step] + w2 * (double)p[2 * step]);
+            const T* p = y + c0 + f * g.nvp;
+            y[o + f * g.nvp] = (T)(w0 * (double)p[0] + w1 * (double)p[step] + w2 * (double)p[2 * step]);
         }
     }
 }
@@ -1417,9 +1416,10 @@ hipError_t launch_advance_time(double* t_dev, double dt, hipStream_t s) {
                                        const double*, double, uint64_t, T*, hipStream_t);                \
     template hipError_t launch_analysis<T>(const Geometry&, const T*, const double*, const double*,      \
                                            double*, double*, hipStream_t);                               \
-    template hipError_t launch_ghosted<T>(const Geometry&, int, int, const T*, double*, hipStream_t);    \
-    template hipError_t launch_pack<T>(const Geometry&, const T*, double*, hipStream_t);                 \
-    template hipError_t launch_unpack<T>(const Geometry&, const double*, T*, hipStream_t);               \
+    template hipError_t launch_ghosted<T>(const Geometry&, int, int, const T*, int, double*, hipStream_t); \
+    template hipError_t launch_pack<T>(const Geometry&, const T*, int, double*, hipStream_t);            \
+    template hipError_t launch_unpack<T>(const Geometry&, const double*, T*, int, hipStream_t);          \
+    template hipError_t launch_xghosts<T>(const Geometry&, T*, hipStream_t);                              \
     template hipError_t launch_p3_slice<T>(const Geometry&, const T*, const double*, const int*, int,    \
                                            const int*, int, double*, hipStream_t);                       \
     template hipError_t launch_renorm<T>(const Geometry&, T*, const double*, const double*, hipStream_t); \
