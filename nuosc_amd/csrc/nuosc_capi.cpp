@@ -86,6 +86,12 @@ int fail(nuosc_sim* s, int code, const std::string& msg) {
 
 bool is64(const nuosc_sim* s) { return s->cfg.precision == NUOSC_PREC_FP64; }
 
+// elements between stored x rows of a state copy: the row's site records
+long long x_row_stride(int nzs, int nvp, int esize) {
+    (void)esize;
+    return (long long)nzs * kNVar * nvp;
+}
+
 void drop_graph(nuosc_sim* s) {
     if (s->graph_exec) hipGraphExecDestroy(s->graph_exec);
     if (s->graph) hipGraphDestroy(s->graph);
@@ -337,7 +343,7 @@ int nuosc_create(const nuosc_config* cfg, nuosc_sim** out) {
         gq.nz = s->nz;
         gq.nch = s->nch;
         gq.nxs = s->nx + 2 * kXg + 1;
-        gq.xrow = (long long)nzs * kNVar * s->nvp;
+        gq.xrow = x_row_stride(nzs, s->nvp, (int)s->esize);
         if (!buffer_addressable(gq, (int)s->esize)) {
             delete s;
             return fail(nullptr, NUOSC_EINVAL,
@@ -393,7 +399,7 @@ int nuosc_create(const nuosc_config* cfg, nuosc_sim** out) {
     s->geo.lpr = lpr;
     s->geo.rows = rows_per_wave((int)s->esize);
     s->geo.nxs = s->nx + 2 * kXg + 1;
-    s->geo.xrow = (long long)nzs * kNVar * s->nvp;
+    s->geo.xrow = x_row_stride(nzs, s->nvp, (int)s->esize);
     s->geo.twelve_dx = 12 * c.dx;
     s->geo.twelve_dz = 12 * c.dz;
     s->geo.ko_x = -c.ko / c.dx / 16.0;  // nuosc_class.cpp:120-121
@@ -1161,7 +1167,7 @@ int nuosc_estimate_device_bytes(const nuosc_config* c, long long* bytes) {
     const long long align = 128 / esize, nvp = (nv + align - 1) / align * align;
     const long long lpr = dirs_per_chunk(esize), nch = (nv + lpr - 1) / lpr;
     const long long nzs = nz + (c->bc == NUOSC_BC_OPEN_Z ? 2 * c->gz : 0);
-    long long b = 4ll * (nx + 2 * kXg + 1) * nzs * kNVar * nvp * esize;   // u, w, yA, yB (+ x ghost, spare rows)
+    long long b = 4ll * (nx + 2 * kXg + 1) * x_row_stride((int)nzs, (int)nvp, esize) * esize;   // u, w, yA, yB
     b += (nx + 1) * nz * nch * kMomRec * 8 + 3ll * nx * nz * kMomStride * 8;  // moments
     b += 4 * nv * 8 + 2 * nv * 8 + 2048ll * kNAna * 16 + kNAna * 8;    // v-grid, G0, analysis
     b += 2 * nz * 8;                                                   // init tables

@@ -51,7 +51,7 @@ struct Geometry {
     int rows;              // x rows per stage-kernel wavefront (rows_per_wave)
     int nch;               // direction chunks per site = ceil(nv / lpr)
     int nxs;               // stored x rows: nx + 2 kXg + 1 (ghost rows and the spare row)
-    long long xrow;        // elements per stored x row: nzs * 8 * nvp
+    long long xrow;        // elements per stored x row: nzs * 8 * nvp + padding (x-row stride)
     double twelve_dx, twelve_dz;   // 12*dx, 12*dz (advection denominators)
     double ko_x, ko_z;     // -ko/dx/16, -ko/dz/16
     const double *vx, *vy, *vz, *vw;   // device, length nv
@@ -62,7 +62,7 @@ struct Geometry {
 // element offset of field f, stored z row js, direction 0 of interior x row i
 // (i = -kXg .. nx + kXg; nx + kXg is the spare row)
 __host__ __device__ inline long long elem_off(const Geometry& g, long long i, long long js, int f) {
-    return ((i + kXg) * g.nzs + js) * (long long)(kNVar * g.nvp) + (long long)f * g.nvp;
+    return (i + kXg) * g.xrow + (js * kNVar + f) * (long long)g.nvp;
 }
 // element offset of direction 0, field 0 of interior site (i * nz + j)
 __host__ __device__ inline long long site_off(const Geometry& g, long long site) {

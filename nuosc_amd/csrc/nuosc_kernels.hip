@@ -195,6 +195,23 @@ struct Rows {
     }
 };
 
+// Value of lane n (n < 16) of x's 16-lane DPP row, in every lane of the row
+// (v_mov_b32_dpp row_newbcast:n on each half).
+template <int N>
+__device__ __forceinline__ double row_bcast(double x) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, x);
+    const unsigned lo = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)b, 0x150 + N, 0xF, 0xF, false);
+    const unsigned hi = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)(b >> 32), 0x150 + N, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+// Projection m0 - m1 vx - m2 vy - m3 vz of moments K..K+3 of a site onto a
+// direction (INP, nuosc_class.cpp:187); the site's moments are held one per
+// lane (lane n of each DPP row holds moment n).
+template <int K>
+__device__ __forceinline__ double row_project(double x, double vx, double vy, double vz) {
+    return row_bcast<K>(x) - row_bcast<K + 1>(x) * vx - row_bcast<K + 2>(x) * vy - row_bcast<K + 3>(x) * vz;
+}
+
 // constant address space: uniform loads from it become scalar loads
 typedef const __attribute__((address_space(4))) double cdouble;
 
@@ -309,7 +326,7 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& a, int lblock, in
     const int nzs = a.g.nzs, zoff = a.g.zoff;             // stored rows per x row, ghost rows below z = 0
     const unsigned rowb = (unsigned)a.g.nvp * sizeof(T);  // bytes per field row
     const unsigned siteb = rowb * kNVar;                  // bytes per site record (8 field rows)
-    const unsigned xrowb = (unsigned)nzs * siteb;         // bytes per stored x row
+    const unsigned xrowb = (unsigned)(a.g.xrow * (long long)sizeof(T));   // x-row stride in bytes
     // The four waves of a block sweep 4R adjacent x rows of one chunk side by
     // side, so most halo rows a wave reads are rows its neighbours in the
     // block read at about the same time (L2 hits).  Chunk fastest across
@@ -381,29 +398,26 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& a, int lblock, in
     const T mu2 = (T)a.mu2, mu = (T)a.mu, p2st = (T)a.p2st, p2ct = (T)a.p2ct;
     const T pmo = (T)a.pmo, ct2 = (T)a.ct2, st = (T)a.st;
     const T cw = (T)a.cw, cy = (T)a.cy, ca = (T)a.ca, cb = (T)a.cb;
-    // moment records: interior sites; a lane past the last row reads row nx-1
+    // moment totals of the lane's site (a lane past the last row reads row
+    // nx-1): lane dl holds value dl % 16 of its row's record; every data row
+    // (16 or 32 lanes) is made of whole 16-lane DPP rows, so row_project()
+    // reaches all 12 values of the lane's site
     const unsigned mbase = (unsigned)i0 * nz;             // record of (i0, z = 0)
-    const unsigned vom = (unsigned)(min(r, nx - 1) - i0) * (unsigned)nz * (kMomStride * 8);
+    const unsigned vom = (unsigned)(min(r, nx - 1) - i0) * (unsigned)nz * (kMomStride * 8) + (unsigned)(dl & 15) * 8;
 
     const int j0 = zs * a.zlen;
     const int j1 = min(nz, j0 + a.zlen);
 
     T win[kNVar][5][V];   // own row, z = j-2 .. j+2
     T hreg[NH][kNVar][V]; // halo rows of z = j+2, staged into LDS at the end of step j
-    // velocity moments of the stage input at this lane's site, z = j (M) and
-    // z = j+1 (Mn, in flight during step j)
-    double M[kNMom], Mn[kNMom];
+    // velocity moments of the stage input at this lane's site (one value per
+    // lane), z = j (Ml) and z = j+1 (Mn, in flight during step j)
     unsigned long long bm = a.mtot;
     asm volatile("" : "+s"(bm));
     const __amdgpu_buffer_rsrc_t rm = __builtin_amdgcn_make_buffer_rsrc((void*)bm, 0, (int)a.mtot_bytes, kRsrcWord3);
-    auto load_moments = [&](double (&dst)[kNMom], int jj) {
+    auto load_moment = [&](int jj) {
         const unsigned sm = (mbase + jj) * (kMomStride * 8);
-#pragma unroll
-        for (int s = 0; s < kNMom; s += 2) {
-            const u32x4 x = __builtin_amdgcn_raw_buffer_load_b128(rm, vom + s * 8, sm, 0);
-            dst[s] = __builtin_bit_cast(double, u32x2{x[0], x[1]});
-            dst[s + 1] = __builtin_bit_cast(double, u32x2{x[2], x[3]});
-        }
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rm, vom, sm, 0));
     };
 #pragma unroll
     for (int qz = 0; qz < 4; ++qz) {
@@ -425,7 +439,7 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& a, int lblock, in
                 for (int e = 0; e < V; ++e) xs[(j0 + qz) & 1][f][slot_h[h] + e] = x[e];
             }
     }
-    load_moments(M, j0);
+    double Ml = load_moment(j0);
 
     for (int j = j0; j < j1; ++j) {
         const unsigned so = (unsigned)(j + zoff) * siteb;
@@ -443,33 +457,40 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& a, int lblock, in
         const unsigned s2 = zrow(j + 2) * siteb;
 #pragma unroll
         for (int f = 0; f < kNVar; ++f) yin.ldv(by, f, vo, s2, win[f][4]);
+        // (scheduling barriers keep the groups in this issue order, so each
+        // wait below covers only the groups needed so far)
+        __builtin_amdgcn_sched_barrier(0);
         T uc[kNVar][V], wc[kNVar][V];
         if constexpr (STAGE == 2 || STAGE == 3 || STAGE == 6 || STAGE == 7) {
             const unsigned long long bu = uu.fresh_base();
 #pragma unroll
             for (int f = 0; f < kNVar; ++f) uu.template ldv<2>(bu, f, vo, so, uc[f]);
+            __builtin_amdgcn_sched_barrier(0);
         }
         if constexpr (STAGE >= 2 && STAGE <= 4) {
             const unsigned long long bw = ww.fresh_base();
 #pragma unroll
             for (int f = 0; f < kNVar; ++f) ww.template ldv<2>(bw, f, vo, so, wc[f]);
+            __builtin_amdgcn_sched_barrier(0);
         }
 #pragma unroll
         for (int h = 0; h < NH; ++h)
 #pragma unroll
             for (int f = 0; f < kNVar; ++f) yin.ldv(by, f, vh[h], s2, hreg[h][f]);
+        __builtin_amdgcn_sched_barrier(0);
         // unconditional (a branch here would make every later wait a full
         // drain); the last row of the box reloads itself
-        load_moments(Mn, min(j + 1, nz - 1));
+        const double Mn = load_moment(min(j + 1, nz - 1));
+        __builtin_amdgcn_sched_barrier(0);
         wave_sync();   // other lanes' writes are visible to the reads below
 
         T yo[kNVar][V], wn[kNVar][V];
 #pragma unroll
         for (int e = 0; e < V; ++e) {
             // projections onto direction v (INP, nuosc_class.cpp:187)
-            const T A = (T)(M[0] - M[1] * vxd[e] - M[2] * vyd[e] - M[3] * vzd[e]);
-            const T B = (T)(M[4] - M[5] * vxd[e] - M[6] * vyd[e] - M[7] * vzd[e]);
-            const T C = (T)(M[8] - M[9] * vxd[e] - M[10] * vyd[e] - M[11] * vzd[e]);
+            const T A = (T)row_project<0>(Ml, vxd[e], vyd[e], vzd[e]);
+            const T B = (T)row_project<4>(Ml, vxd[e], vyd[e], vzd[e]);
+            const T C = (T)row_project<8>(Ml, vxd[e], vyd[e], vzd[e]);
 
             const T ee = win[0][2][e], mm = win[1][2][e], emr = win[2][2][e], emi = win[3][2][e];
             const T bee = win[4][2][e], bmm = win[5][2][e], bemr = win[6][2][e], bemi = win[7][2][e];
@@ -577,8 +598,7 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& a, int lblock, in
                 win[f][2][e] = win[f][3][e];
                 win[f][3][e] = win[f][4][e];
             }
-#pragma unroll
-        for (int s = 0; s < kNMom; ++s) M[s] = Mn[s];
+        Ml = Mn;
         // halo rows of z = j+2 (loaded in this iteration) into this level,
         // whose z = j reads are done; level lv ^ 1 holds z = j+1.  Done here
         // rather than at the top of the next iteration: a wait at the loop
