@@ -201,11 +201,13 @@ int nuosc_last_step_ms(nuosc_sim* sim, double* ms);
 int nuosc_profile_stages(nuosc_sim* sim, int nsteps, double* stage_ms);
 
 /* Stage-kernel decomposition (no reference counterpart).  Each wavefront
- * marches along z over a segment of zlen sites; nuosc_autotune times the
- * RK4 stage-1 kernel on this handle's own grid for zlen in {8, ..., 256}
- * (trials launches each, the fastest kept) and keeps the best.  Call it at
- * a step boundary: it overwrites only the RK4 scratch copies, never the
- * state.  Results are cached per (nx, nz, nv, precision) for the process. */
+ * marches along z over a segment of zlen sites; nuosc_autotune times RK4
+ * stages 1-3 (with their boundary fills and moment totals) on this handle's
+ * own grid for zlen in {8, ..., 256}, the candidates in turn for trials
+ * rounds after a warm-up round, keeps each candidate's fastest time and
+ * selects the best.  Call it at a step boundary: it overwrites only the RK4
+ * scratch copies, never the state.  Results are cached per (nx, nz, nv,
+ * precision) for the process. */
 int nuosc_get_plan(const nuosc_sim* sim, int* zlen, int* waves);
 int nuosc_set_zlen(nuosc_sim* sim, int zlen);
 int nuosc_autotune(nuosc_sim* sim, int trials, int* best_zlen, double* best_ms);

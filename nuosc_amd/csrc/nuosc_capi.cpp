@@ -29,7 +29,8 @@ struct nuosc_sim {
     nuosc_config cfg{};
     // geometry (CartGrid.cpp:191-208)
     int nx = 0, nz = 0, nv = 0, nvp = 0, nch = 0;
-    long long copy_elems = 0;   // elements per state copy
+    long long srec = 0, xrow = 0;   // site-record and x-row strides of the state copies (elements)
+    long long state_elems = 0;      // elements allocated for the four copies
     double dt = 0, ds_L = 0, ct = 0, st = 0;
     double phy_time = 0;
     double n_nue0[2] = {0, 0};
@@ -86,10 +87,24 @@ int fail(nuosc_sim* s, int code, const std::string& msg) {
 
 bool is64(const nuosc_sim* s) { return s->cfg.precision == NUOSC_PREC_FP64; }
 
-// elements between stored x rows of a state copy: the row's site records
-long long x_row_stride(int nzs, int nvp, int esize) {
-    (void)esize;
-    return (long long)nzs * kNVar * nvp;
+// Strides of the four state copies (separate allocations, site records of
+// 8 nvp elements); ok = false: the stage kernel's per-wave range of x rows
+// or the moment records do not fit 32-bit buffer offsets.
+struct CopyLayout {
+    long long srec, xrow, elems;   // strides and elements allocated for all four copies
+    bool ok;
+};
+CopyLayout copy_layout(int nx, int nz, int nzs, int nvp, int nch, int esize) {
+    Geometry gq{};
+    gq.nx = nx;
+    gq.nz = nz;
+    gq.nch = nch;
+    gq.nvp = nvp;
+    gq.nxs = nx + 2 * kXg + 1;
+    gq.srec = (long long)kNVar * nvp;
+    gq.xrow = (long long)nzs * gq.srec;
+    if (!buffer_addressable(gq, esize)) return CopyLayout{0, 0, 0, false};
+    return CopyLayout{gq.srec, gq.xrow, 4 * copy_elems(gq), true};
 }
 
 void drop_graph(nuosc_sim* s) {
@@ -265,6 +280,35 @@ int check_field(nuosc_sim* s, int f) {
     return NUOSC_OK;
 }
 
+// RK4 stages 1-3 with their boundary fills and moment totals: they write only
+// the scratch copies w, yA, yB and the scratch moment records, never u.
+template <typename T>
+hipError_t enqueue_scratch_stages(nuosc_sim* s) {
+    const double dt = s->dt, c6 = dt / 6, c3 = dt / 3, c2 = dt / 2;
+    const Geometry& g = s->geo;
+    T* u = static_cast<T*>(s->st_u);
+    T* w = static_cast<T*>(s->st_w);
+    T* ya = static_cast<T*>(s->st_ya);
+    T* yb = static_cast<T*>(s->st_yb);
+    const bool comp = sizeof(T) == 4, open = g.bc == NUOSC_BC_OPEN_Z;
+    double* P = s->mom_part;
+    hipError_t e = hipSuccess;
+    auto ghosts = [&](T* y, double cdt) {
+        if (e == hipSuccess && open)
+            e = launch_open_ghosts<T>(g, y, s->d_g0, s->d_g0 + s->nv, s->inflow, s->d_time, cdt, s->stream);
+        if (e == hipSuccess) e = launch_xghosts<T>(g, y, s->stream);
+    };
+    auto run = [&](hipError_t r) { if (e == hipSuccess) e = r; };
+    ghosts(u, 0.0);
+    run(launch_stage<T>(1, g, s->ph, s->plan, u, nullptr, w, ya, s->mom_u, P, c6, c2, s->stream));
+    run(launch_moment_totals(g, comp, P, s->mom_a, s->stream));
+    ghosts(ya, c2);
+    run(launch_stage<T>(2, g, s->ph, s->plan, ya, u, w, yb, s->mom_a, P, c3, c2, s->stream));
+    run(launch_moment_totals(g, comp, P, s->mom_b, s->stream));
+    ghosts(yb, c2);
+    run(launch_stage<T>(3, g, s->ph, s->plan, yb, u, w, ya, s->mom_b, P, c3, dt, s->stream));
+    return e;
+}
 }  // namespace
 
 extern "C" {
@@ -338,19 +382,16 @@ int nuosc_create(const nuosc_config* cfg, nuosc_sim** out) {
     const int lpr = dirs_per_chunk((int)s->esize);
     s->nch = (s->nv + lpr - 1) / lpr;
     {
-        Geometry gq{};
-        gq.nx = s->nx;
-        gq.nz = s->nz;
-        gq.nch = s->nch;
-        gq.nxs = s->nx + 2 * kXg + 1;
-        gq.xrow = x_row_stride(nzs, s->nvp, (int)s->esize);
-        if (!buffer_addressable(gq, (int)s->esize)) {
+        const CopyLayout L = copy_layout(s->nx, s->nz, nzs, s->nvp, s->nch, (int)s->esize);
+        if (!L.ok) {
             delete s;
             return fail(nullptr, NUOSC_EINVAL,
                         "grid too large for one GPU handle: a few stored x rows (nz * nv) or the moment records "
                         "(nx * nz * nv) exceed 4 GiB");
         }
-        s->copy_elems = copy_elems(gq);
+        s->srec = L.srec;
+        s->xrow = L.xrow;
+        s->state_elems = L.elems;
     }
     // nuosc_class.cpp:461-464, nuosc_class.h:120-122
     s->ds_L = c.dx * c.dz / (c.z1 - c.z0) / (c.x1 - c.x0);
@@ -371,14 +412,17 @@ int nuosc_create(const nuosc_config* cfg, nuosc_sim** out) {
     if (hipEventCreate(&s->ev0) != hipSuccess || hipEventCreate(&s->ev1) != hipSuccess)
         return bail(NUOSC_EHIP, "event creation failed");
 
-    const size_t state_bytes = (size_t)s->copy_elems * s->esize;
+    const size_t state_bytes = (size_t)s->state_elems * s->esize;
     const size_t part_bytes = (size_t)(s->nx + 1) * s->nz * s->nch * kMomRec * sizeof(double);
     const size_t mom_bytes = (size_t)s->nx * s->nz * kMomStride * sizeof(double);
     void** states[4] = {&s->st_u, &s->st_w, &s->st_ya, &s->st_yb};
-    for (void** p : states) {
-        if (hipMalloc(p, state_bytes) != hipSuccess) return bail(NUOSC_ENOMEM, "state allocation failed");
-        s->device_bytes += (long long)state_bytes;
-        if (hipMemsetAsync(*p, 0, state_bytes, s->stream) != hipSuccess) return bail(NUOSC_EHIP, "memset failed");
+    {
+        const size_t copy_bytes = state_bytes / 4;
+        for (void** p : states) {
+            if (hipMalloc(p, copy_bytes) != hipSuccess) return bail(NUOSC_ENOMEM, "state allocation failed");
+            s->device_bytes += (long long)copy_bytes;
+            if (hipMemsetAsync(*p, 0, copy_bytes, s->stream) != hipSuccess) return bail(NUOSC_EHIP, "memset failed");
+        }
     }
     double** moms[4] = {&s->mom_part, &s->mom_u, &s->mom_a, &s->mom_b};
     for (double** p : moms) {
@@ -399,7 +443,8 @@ int nuosc_create(const nuosc_config* cfg, nuosc_sim** out) {
     s->geo.lpr = lpr;
     s->geo.rows = rows_per_wave((int)s->esize);
     s->geo.nxs = s->nx + 2 * kXg + 1;
-    s->geo.xrow = x_row_stride(nzs, s->nvp, (int)s->esize);
+    s->geo.srec = s->srec;
+    s->geo.xrow = s->xrow;
     s->geo.twelve_dx = 12 * c.dx;
     s->geo.twelve_dz = 12 * c.dz;
     s->geo.ko_x = -c.ko / c.dx / 16.0;  // nuosc_class.cpp:120-121
@@ -725,6 +770,7 @@ std::mutex g_tune_mu;
 std::map<std::tuple<int, int, int, int>, int> g_tune_cache;
 }  // namespace
 
+
 int nuosc_autotune(nuosc_sim* s, int trials, int* best_zlen, double* best_ms) {
     if (!s || trials < 1) return NUOSC_EINVAL;
     const auto key = std::make_tuple(s->nx, s->nz, s->nv, s->cfg.precision);
@@ -745,45 +791,41 @@ int nuosc_autotune(nuosc_sim* s, int trials, int* best_zlen, double* best_ms) {
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
         return fail(s, NUOSC_EHIP, "event creation failed");
-    {
-        const int rc = fill_state_ghosts(s, s->st_u);   // stage 1 reads u
-        if (rc) return rc;
-    }
     const StagePlan keep = s->plan;
-    int best = keep.zlen;
-    float bms = 1e30f;
-    const double dt = s->dt;
+    std::vector<int> cand;
     for (int zl = 8; zl <= 256; zl *= 2) {
         if (zl > s->nz && zl != 8) break;
-        s->plan = plan_stage_zlen(s->geo, zl);
-        float tmin = 1e30f;
-        for (int t = 0; t < trials + 1; ++t) {  // first launch warms up
-            hipError_t le;
+        cand.push_back(zl);
+    }
+    std::vector<float> tmin(cand.size(), 1e30f);
+    // candidates in turn, trials + 1 rounds (the first warms up), so a clock
+    // ramp or drift during the search favours none of them
+    int rc = NUOSC_OK;
+    for (int t = 0; t < trials + 1 && rc == NUOSC_OK; ++t)
+        for (size_t k = 0; k < cand.size() && rc == NUOSC_OK; ++k) {
+            s->plan = plan_stage_zlen(s->geo, cand[k]);
             (void)hipEventRecord(e0, s->stream);
-            if (is64(s))
-                le = launch_stage<double>(1, s->geo, s->ph, s->plan, (double*)s->st_u, nullptr, (double*)s->st_w,
-                                          (double*)s->st_ya, s->mom_u, s->mom_part, dt / 6, dt / 2, s->stream);
-            else
-                le = launch_stage<float>(1, s->geo, s->ph, s->plan, (float*)s->st_u, nullptr, (float*)s->st_w,
-                                         (float*)s->st_ya, s->mom_u, s->mom_part, dt / 6, dt / 2, s->stream);
+            const hipError_t le =
+                is64(s) ? enqueue_scratch_stages<double>(s) : enqueue_scratch_stages<float>(s);
             (void)hipEventRecord(e1, s->stream);
             if (le != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
-                (void)hipEventDestroy(e0);
-                (void)hipEventDestroy(e1);
-                s->plan = keep;
-                return fail(s, NUOSC_EHIP, "autotune launch failed");
+                rc = fail(s, NUOSC_EHIP, "autotune launch failed");
+                break;
             }
             float ms = 0;
             (void)hipEventElapsedTime(&ms, e0, e1);
-            if (t > 0) tmin = std::min(tmin, ms);
+            if (t > 0) tmin[k] = std::min(tmin[k], ms);
         }
-        if (tmin < bms) {
-            bms = tmin;
-            best = zl;
-        }
-    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    if (rc) {
+        s->plan = keep;
+        return rc;
+    }
+    size_t kb = 0;
+    for (size_t k = 1; k < cand.size(); ++k)
+        if (tmin[k] < tmin[kb]) kb = k;
+    const int best = cand[kb];
     s->plan = plan_stage_zlen(s->geo, best);
     drop_graph(s);
     {
@@ -791,7 +833,7 @@ int nuosc_autotune(nuosc_sim* s, int trials, int* best_zlen, double* best_ms) {
         g_tune_cache[key] = best;
     }
     if (best_zlen) *best_zlen = best;
-    if (best_ms) *best_ms = bms;
+    if (best_ms) *best_ms = tmin[kb];
     return NUOSC_OK;
 }
 
@@ -1122,14 +1164,15 @@ int nuosc_batch_autotune(nuosc_batch* b, int trials, int* best_zlen, double* bes
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
         return bfail(b, NUOSC_EHIP, "event creation failed");
     const int keep = b->zlen;
-    int best = keep;
-    float bms = 1e30f;
-    for (int zl = 16; zl <= 256; zl *= 2) {
-        b->zlen = zl;
-        rc = batch_tables(b);
-        if (rc) break;
-        float tmin = 1e30f;
-        for (int t = 0; t < trials + 1 && rc == NUOSC_OK; ++t) {  // first launch warms up
+    const std::vector<int> cand = {16, 32, 64, 128, 256};
+    std::vector<float> tmin(cand.size(), 1e30f);
+    // candidates in turn (tables rebuilt per candidate), trials + 1 rounds,
+    // the first a warm-up, so clock drift during the search favours none
+    for (int t = 0; t < trials + 1 && rc == NUOSC_OK; ++t)
+        for (size_t k = 0; k < cand.size() && rc == NUOSC_OK; ++k) {
+            b->zlen = cand[k];
+            rc = batch_tables(b);
+            if (rc) break;
             (void)hipEventRecord(e0, b->stream);
             // stage 1 writes only the RK4 scratch copies (w, yA), never the state
             hipError_t le = batch_stage(b->tables, 1, b->stream);
@@ -1140,21 +1183,19 @@ int nuosc_batch_autotune(nuosc_batch* b, int trials, int* best_zlen, double* bes
             }
             float ms = 0;
             (void)hipEventElapsedTime(&ms, e0, e1);
-            if (t > 0) tmin = std::min(tmin, ms);
+            if (t > 0) tmin[k] = std::min(tmin[k], ms);
         }
-        if (rc == NUOSC_OK && tmin < bms) {
-            bms = tmin;
-            best = zl;
-        }
-    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    b->zlen = rc == NUOSC_OK ? best : keep;
+    size_t kb = 0;
+    for (size_t k = 1; k < cand.size(); ++k)
+        if (tmin[k] < tmin[kb]) kb = k;
+    b->zlen = rc == NUOSC_OK ? cand[kb] : keep;
     b->dirty = true;
     batch_drop_graph(b);
     if (rc) return rc;
-    if (best_zlen) *best_zlen = best;
-    if (best_ms) *best_ms = bms;
+    if (best_zlen) *best_zlen = cand[kb];
+    if (best_ms) *best_ms = tmin[kb];
     return NUOSC_OK;
 }
 
@@ -1167,7 +1208,9 @@ int nuosc_estimate_device_bytes(const nuosc_config* c, long long* bytes) {
     const long long align = 128 / esize, nvp = (nv + align - 1) / align * align;
     const long long lpr = dirs_per_chunk(esize), nch = (nv + lpr - 1) / lpr;
     const long long nzs = nz + (c->bc == NUOSC_BC_OPEN_Z ? 2 * c->gz : 0);
-    long long b = 4ll * (nx + 2 * kXg + 1) * x_row_stride((int)nzs, (int)nvp, esize) * esize;   // u, w, yA, yB
+    const CopyLayout L = copy_layout((int)nx, (int)nz, (int)nzs, (int)nvp, (int)nch, esize);
+    if (!L.ok) return NUOSC_EINVAL;
+    long long b = L.elems * esize;                                     // u, w, yA, yB
     b += (nx + 1) * nz * nch * kMomRec * 8 + 3ll * nx * nz * kMomStride * 8;  // moments
     b += 4 * nv * 8 + 2 * nv * 8 + 2048ll * kNAna * 16 + kNAna * 8;    // v-grid, G0, analysis
     b += 2 * nz * 8;                                                   // init tables

@@ -1,13 +1,14 @@
 // nuosc_dev.h -- internal interface between the C-ABI layer (nuosc_capi.cpp)
 // and the HIP kernels (nuosc_kernels.hip).  Not part of the public ABI.
 //
-// Device data layout (one state copy):
+// Device data layout (state copies u, w, yA, yB):
 //   site records of 8 fields (ee, mm, emr, emi, bee, bmm, bemr, bemi), each
 //   field a row of nvp directions (v fastest; nvp = nv rounded up so that
 //   every field row starts on a 128-byte boundary):
 //     copy[xs][zs][f][nvp],  xs = i + kXg (stored x row), zs = j + zoff
-//   so the 8 fields of a site are one contiguous 8*nvp block and a stored x
-//   row (nzs site records) is contiguous.  Stored x rows: kXg = 2 periodic
+//   so the 8 fields of a site are one contiguous block (srec = 8 nvp elements
+//   between site records) and a stored x row (nzs records) is contiguous.
+//   Each copy is its own allocation.  Stored x rows: kXg = 2 periodic
 //   ghost rows at each end (copies of interior rows nx-2, nx-1 and 0, 1,
 //   refreshed before every stage from the interior) and one spare row at
 //   xs = nx + 2 kXg, where stage-kernel lanes past the last x row store
@@ -51,7 +52,8 @@ struct Geometry {
     int rows;              // x rows per stage-kernel wavefront (rows_per_wave)
     int nch;               // direction chunks per site = ceil(nv / lpr)
     int nxs;               // stored x rows: nx + 2 kXg + 1 (ghost rows and the spare row)
-    long long xrow;        // elements per stored x row: nzs * 8 * nvp + padding (x-row stride)
+    long long srec;        // elements between consecutive site records of one copy (8 nvp)
+    long long xrow;        // elements between stored x rows of one copy: nzs * srec
     double twelve_dx, twelve_dz;   // 12*dx, 12*dz (advection denominators)
     double ko_x, ko_z;     // -ko/dx/16, -ko/dz/16
     const double *vx, *vy, *vz, *vw;   // device, length nv
@@ -62,13 +64,13 @@ struct Geometry {
 // element offset of field f, stored z row js, direction 0 of interior x row i
 // (i = -kXg .. nx + kXg; nx + kXg is the spare row)
 __host__ __device__ inline long long elem_off(const Geometry& g, long long i, long long js, int f) {
-    return (i + kXg) * g.xrow + (js * kNVar + f) * (long long)g.nvp;
+    return (i + kXg) * g.xrow + js * g.srec + (long long)f * g.nvp;
 }
 // element offset of direction 0, field 0 of interior site (i * nz + j)
 __host__ __device__ inline long long site_off(const Geometry& g, long long site) {
     return elem_off(g, site / g.nz, site % g.nz + g.zoff, 0);
 }
-// elements per state copy
+// elements spanned by one state copy (from its first element)
 inline long long copy_elems(const Geometry& g) { return (long long)g.nxs * g.xrow; }
 
 // Inflow data of the open z boundary: the ipt-0 pulse

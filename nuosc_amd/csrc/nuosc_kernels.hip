@@ -309,8 +309,8 @@ struct Slab {
     T v[2][kNVar][NS * D];   // two z levels (by parity) of all 8 fields
 };
 
-// One wavefront's work.  Block lblock = (direction chunk, group of 4
-// x-row quads, z segment), chunk fastest; wave wib of the block takes quad
+// One wavefront's work.  Block lblock = (group of 4 x-row quads, direction
+// chunk, z segment), quad group fastest; wave wib of the block takes quad
 // 4 * group + wib.  Lane (g, dl) of a wave handles x row i0 + g and the V
 // adjacent directions ch * D + dl * V + e, e < V.
 template <typename T, int R, int STAGE, bool COMP>
@@ -325,18 +325,20 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& a, int lblock, in
     const int nx = a.g.nx, nz = a.g.nz, nch = a.g.nch, nv = a.g.nv;
     const int nzs = a.g.nzs, zoff = a.g.zoff;             // stored rows per x row, ghost rows below z = 0
     const unsigned rowb = (unsigned)a.g.nvp * sizeof(T);  // bytes per field row
-    const unsigned siteb = rowb * kNVar;                  // bytes per site record (8 field rows)
+    const unsigned siteb = (unsigned)(a.g.srec * (long long)sizeof(T));   // site-record stride in bytes
     const unsigned xrowb = (unsigned)(a.g.xrow * (long long)sizeof(T));   // x-row stride in bytes
     // The four waves of a block sweep 4R adjacent x rows of one chunk side by
     // side, so most halo rows a wave reads are rows its neighbours in the
-    // block read at about the same time (L2 hits).  Chunk fastest across
-    // blocks: neighbouring blocks, which the XCD remap keeps on one XCD, read
-    // adjacent 128-byte segments of the same site records.
-    const int ch = lblock % nch;
-    const int rest = lblock / nch;
+    // block read at about the same time (L2 hits).  Quad group fastest across
+    // blocks: the neighbouring blocks, which the XCD remap keeps on one XCD
+    // and starts at about the same time, own the block's edge halo rows.
+    // (Chunk fastest, which keeps the chunks of a site record together,
+    // measured 3-7% slower.)
     const int nqg = (a.nquads + 3) >> 2;
-    const int qd = (rest % nqg) * 4 + wib;
-    const int zs = rest / nqg;
+    const int qd = (lblock % nqg) * 4 + wib;
+    const int rest = lblock / nqg;
+    const int ch = rest % nch;
+    const int zs = rest / nch;
     if (qd >= a.nquads) return;   // last group of a grid with nquads % 4 != 0
     const int i0 = qd * R;
     const int g = lane / L, dl = lane % L;
@@ -742,12 +744,23 @@ hipError_t launch_stage(int stage, const Geometry& g, const Physics& ph, const S
 // ---------------------------------------------------------------------------
 struct XgArgs {
     unsigned long long y;   // state copy
-    long long words;        // 16-byte words per stored x row
+    long long words;        // 16-byte words of one copy per stored x row (nzs records)
+    long long rowstride;    // 16-byte words between stored x rows
+    int recw, recstride;    // 16-byte words of one site record, between site records
     int nx;
 };
 constexpr int kXgPerThread = 4;
 
-long long xg_words(const Geometry& g, int esize) { return g.xrow * esize / 16; }
+XgArgs xg_args(const Geometry& g, const void* y, int esize) {
+    XgArgs a;
+    a.y = (unsigned long long)y;
+    a.recw = kNVar * g.nvp * esize / 16;
+    a.recstride = (int)(g.srec * esize / 16);
+    a.words = (long long)g.nzs * a.recw;
+    a.rowstride = g.xrow * esize / 16;
+    a.nx = g.nx;
+    return a;
+}
 long long xg_blocks(long long words) { return 4 * ((words + 256 * kXgPerThread - 1) / (256 * kXgPerThread)); }
 
 __device__ __forceinline__ void xg_body(const XgArgs& a, long long blk) {
@@ -756,12 +769,15 @@ __device__ __forceinline__ void xg_body(const XgArgs& a, long long blk) {
     const long long c = blk % per;
     const int dst = k < 2 ? k : a.nx + k;           // stored rows 0, 1, nx+2, nx+3
     const int src = wrapi(dst - kXg, a.nx) + kXg;   // stored row of its interior x row
-    uint4* d = reinterpret_cast<uint4*>(a.y) + dst * a.words;
-    const uint4* sp = reinterpret_cast<const uint4*>(a.y) + src * a.words;
+    uint4* d = reinterpret_cast<uint4*>(a.y) + dst * a.rowstride;
+    const uint4* sp = reinterpret_cast<const uint4*>(a.y) + src * a.rowstride;
 #pragma unroll
     for (int q = 0; q < kXgPerThread; ++q) {
         const long long n = (c * kXgPerThread + q) * 256 + threadIdx.x;
-        if (n < a.words) d[n] = sp[n];
+        if (n < a.words) {
+            const long long o = (n / a.recw) * a.recstride + n % a.recw;   // record, word in record
+            d[o] = sp[o];
+        }
     }
 }
 
@@ -776,7 +792,7 @@ __global__ __launch_bounds__(256) void k_xghosts_batch(const XgArgs* __restrict_
 
 template <typename T>
 hipError_t launch_xghosts(const Geometry& g, T* y, hipStream_t s) {
-    const XgArgs a{(unsigned long long)y, xg_words(g, sizeof(T)), g.nx};
+    const XgArgs a = xg_args(g, y, sizeof(T));
     hipLaunchKernelGGL(k_xghosts, dim3((unsigned)xg_blocks(a.words)), dim3(256), 0, s, a);
     return hipGetLastError();
 }
@@ -862,7 +878,7 @@ static hipError_t batch_build_t(BatchTables& t, const std::vector<BatchMember>& 
         }
     }
     for (int r = 0; r < n; ++r) {
-        const long long nb = xg_blocks(xg_words(ms[r].g, sizeof(T)));
+        const long long nb = xg_blocks(xg_args(ms[r].g, nullptr, sizeof(T)).words);
         for (long long b = 0; b < nb; ++b) {
             xmap.push_back(r);
             xmap.push_back((int)b);
@@ -899,7 +915,7 @@ static hipError_t batch_build_t(BatchTables& t, const std::vector<BatchMember>& 
             }
             args[r] = make_stage_args<T>(m.g, m.ph, m.pl, yin, uu, w, yout, mt, m.mpart, cw, cy);
             tots[r] = TotArgs{m.mpart, mo, m.g.nch, (long long)m.g.nx * m.g.nz};
-            xgs[r] = XgArgs{(unsigned long long)yin, xg_words(m.g, sizeof(T)), m.g.nx};
+            xgs[r] = xg_args(m.g, yin, sizeof(T));
         }
         if ((e = hipMalloc(&t.args[st - 1], n * sizeof(StageArgs<T>))) != hipSuccess) return e;
         if ((e = hipMalloc(&t.tot[st - 1], n * sizeof(TotArgs))) != hipSuccess) return e;
@@ -1398,7 +1414,7 @@ __global__ __launch_bounds__(256) void k_open_ghosts(const Geometry g, T* __rest
         const double w2 = q == 1 ? 1.0 : (q == 2 ? 3.0 : 6.0);
         const int e0 = side == 0 ? 0 : g.nz - 1, dir = side == 0 ? 1 : -1;
         const long long c0 = elem_off(g, i, e0 + g.zoff, 0) + v;
-        const long long step = (long long)dir * kNVar * g.nvp;   // one site record along z
+        const long long step = (long long)dir * g.srec;   // one site record along z
 #pragma unroll
         for (int f = 0; f < kNVar; ++f) {
             const T* p = y + c0 + f * g.nvp;
