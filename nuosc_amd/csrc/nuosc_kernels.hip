@@ -221,13 +221,12 @@ typedef const __attribute__((address_space(4))) double cdouble;
 // A wavefront covers R adjacent x-rows x L = 64/R directions (FP64: 4 x 16,
 // FP32: 2 x 32, so every row segment is one 128-byte line) and marches along
 // z over one segment.  Per lane it keeps a 5-deep z-window of its own row for
-// all 8 fields in registers.  The halo rows (i0-2, i0-1, i0+R, i0+R+1) are
-// loaded at z+2, in the same step in which the neighbouring waves of the
-// block load those rows as their own window rows (so the second read hits in
-// L2), and staged, with the own rows, into a wave-private two-level LDS slab
-// through which the x-neighbours are read (no block barrier: one wave's LDS
-// operations complete in order).  Moments of z+1 are prefetched during step
-// z.
+// all 8 fields in registers.  The four waves of a block cover 4R adjacent
+// rows of one direction chunk and move along z together (one LDS barrier
+// per step): each writes its own rows into a block-shared two-level LDS slab
+// through which all x-neighbours are read, so only the 4 rows beyond the
+// block's edges are loaded as halo rows (one per wave, at z+2, staged into
+// LDS one step later).  Moments of z+1 are prefetched during step z.
 // ---------------------------------------------------------------------------
 template <typename T>
 struct StageArgs {
@@ -301,13 +300,23 @@ __device__ __forceinline__ double group_sum16(const double (&a)[16], const doubl
     }
 }
 
-// x rows staged per wave: i0-2 .. i0+R+1, D = 16 V directions each
+// x rows staged per block: block rows -2 .. 4R+1 (slots 0 .. 4R+3) and one
+// slot that idle lanes write; D = L V directions each
 template <typename T, int R>
 struct Slab {
     static constexpr int D = (kWave / R) * kVals<T>;
-    static constexpr int NS = R + 4;
+    static constexpr int NS = 4 * R + 5;
     T v[2][kNVar][NS * D];   // two z levels (by parity) of all 8 fields
 };
+
+// Orders the LDS accesses of a block: every LDS write of every wave before it
+// is visible to every LDS read after it.  The fences cover LDS only, so
+// outstanding global loads and stores are not waited for.
+__device__ __forceinline__ void block_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
 
 // One wavefront's work.  Block lblock = (group of 4 x-row quads, direction
 // chunk, z segment), quad group fastest; wave wib of the block takes quad
@@ -318,7 +327,6 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& a, int lblock, in
     constexpr int V = kVals<T>;    // directions per lane (one 8-byte load)
     constexpr int L = kWave / R;   // lanes per row
     constexpr int D = L * V;       // directions per row chunk (one 128-byte line)
-    constexpr int NH = 4 / R;      // halo loads per field
     auto& xs = slab.v;
 
     const int lane = threadIdx.x & 63;
@@ -335,12 +343,13 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& a, int lblock, in
     // (Chunk fastest, which keeps the chunks of a site record together,
     // measured 3-7% slower.)
     const int nqg = (a.nquads + 3) >> 2;
-    const int qd = (lblock % nqg) * 4 + wib;
+    const int ib = (lblock % nqg) * 4 * R;   // first x row of the block
     const int rest = lblock / nqg;
     const int ch = rest % nch;
     const int zs = rest / nch;
-    if (qd >= a.nquads) return;   // last group of a grid with nquads % 4 != 0
-    const int i0 = qd * R;
+    // (a wave whose quad lies past the last one still runs, with all its
+    // rows past nx: every wave of the block takes part in each step's barrier)
+    const int i0 = ib + wib * R;
     const int g = lane / L, dl = lane % L;
     const int p = ch * L + dl;           // direction group: directions p*V .. p*V+V-1
     const int r = i0 + g;
@@ -348,36 +357,36 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& a, int lblock, in
     const bool act = row_ok && p * V < nv;   // nv is a multiple of V
     const int pl = min(p, nv / V - 1);       // loads stay inside the directions
 
-    // The wave's x rows i0-2 .. i0+R+1 are the stored rows i0 .. i0+R+3 (the
-    // periodic ghost rows make them contiguous); rows past the last interior
-    // row + 2 map to the spare stored row nx + 4.  One descriptor per state
-    // copy covers stored rows i0 .. min(i0 + R + 4, nxs - 1); a lane's stored
-    // row relative to i0 goes into its vector offset.
-    const int spare = nx + 2 * kXg - i0;     // spare stored row, relative to i0
-    const unsigned long long woff = (unsigned long long)i0 * xrowb;
-    const unsigned wbytes = (unsigned)(min(R + 5, a.g.nxs - i0)) * xrowb;
+    // The block's x rows ib-2 .. ib+4R+1 are the stored rows ib .. ib+4R+3
+    // (the periodic ghost rows make them contiguous); rows past the last
+    // interior row + 2 map to the spare stored row nx + 4.  One descriptor
+    // per state copy covers stored rows ib .. min(ib + 4R + 4, nxs - 1); a
+    // lane's stored row relative to ib goes into its vector offset.
+    const int spare = nx + 2 * kXg - ib;     // spare stored row, relative to ib
+    const unsigned long long woff = (unsigned long long)ib * xrowb;
+    const unsigned wbytes = (unsigned)(min(4 * R + 5, a.g.nxs - ib)) * xrowb;
     const Rows<T> yin{a.yin + woff, wbytes, rowb}, uu{a.u + woff, wbytes, rowb};
     const Rows<T> ww{a.w + woff, wbytes, rowb}, yout{a.yout + woff, wbytes, rowb};
     // stored z row of interior z index jj (periodic: wrapped; open: the ghost
     // rows hold jj in [-zoff, nz + zoff))
     auto zrow = [&](int jj) { return (unsigned)wrapi(jj + zoff, nzs); };
     const unsigned dof = (unsigned)(pl * V) * sizeof(T);
-    const unsigned vo = (unsigned)min(g + kXg, spare) * xrowb + dof;
+    const int own = wib * R + g;             // block row of the lane
+    const unsigned vo = (unsigned)min(own + kXg, spare) * xrowb + dof;
     // Stores are unconditional (no branch, so the compiler can count the
     // outstanding memory operations across iterations exactly): a lane past
     // the last x row writes the spare row, a lane past nv writes its own
     // padding columns; neither is ever read by a lane that stores a result.
-    const unsigned vos = (unsigned)(row_ok ? g + kXg : spare) * xrowb + (unsigned)(p * V) * sizeof(T);
-    unsigned vh[NH];
-    int slot_h[NH];
-#pragma unroll
-    for (int h = 0; h < NH; ++h) {
-        const int k = h * R + g;   // halo index 0..3 -> rows i0-2, i0-1, i0+R, i0+R+1
-        const int hs = k < 2 ? k : min(R + k, spare);   // stored row relative to i0
-        vh[h] = (unsigned)hs * xrowb + dof;
-        slot_h[h] = (k < 2 ? k : R + k) * D + dl * V;
-    }
-    const int slot_c = (g + 2) * D + dl * V;
+    const unsigned vos = (unsigned)(row_ok ? own + kXg : spare) * xrowb + (unsigned)(p * V) * sizeof(T);
+    // Edge rows: wave k loads block row -2, -1, 4R, 4R+1 (k = 0..3) with its
+    // first L lanes.  The other lanes' offset lies past the descriptor's
+    // range, so they fetch nothing (the load returns zeros), and they write
+    // the idle slot.
+    const bool eln = g == 0;
+    const int erow = wib < 2 ? wib : min(4 * R + wib, spare);   // stored row relative to ib
+    const unsigned vh = eln ? (unsigned)erow * xrowb + dof : wbytes;
+    const int slot_h = (eln ? (wib < 2 ? wib : 4 * R + wib) : 4 * R + 4) * D + dl * V;
+    const int slot_c = (own + 2) * D + dl * V;
 
     // per-direction constants (nuosc_class.cpp:119-121, 180-181)
     double vxd[V], vyd[V], vzd[V], q[V][4];
@@ -404,16 +413,16 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& a, int lblock, in
     // nx-1): lane dl holds value dl % 16 of its row's record; every data row
     // (16 or 32 lanes) is made of whole 16-lane DPP rows, so row_project()
     // reaches all 12 values of the lane's site
-    const unsigned mbase = (unsigned)i0 * nz;             // record of (i0, z = 0)
-    const unsigned vom = (unsigned)(min(r, nx - 1) - i0) * (unsigned)nz * (kMomStride * 8) + (unsigned)(dl & 15) * 8;
+    const unsigned mbase = (unsigned)ib * nz;             // record of (ib, z = 0)
+    const unsigned vom = (unsigned)(min(r, nx - 1) - ib) * (unsigned)nz * (kMomStride * 8) + (unsigned)(dl & 15) * 8;
 
     const int j0 = zs * a.zlen;
     const int j1 = min(nz, j0 + a.zlen);
 
     T win[kNVar][5][V];   // own row, z = j-2 .. j+2
-    T hreg[NH][kNVar][V]; // halo rows of z = j+2, staged into LDS at the end of step j
+    T hreg[kNVar][V];     // edge row of z = j+2, staged into LDS in step j+1
     // velocity moments of the stage input at this lane's site (one value per
-    // lane), z = j (Ml) and z = j+1 (Mn, in flight during step j)
+    // lane) for z = j, loaded at the end of step j-1
     unsigned long long bm = a.mtot;
     asm volatile("" : "+s"(bm));
     const __amdgpu_buffer_rsrc_t rm = __builtin_amdgcn_make_buffer_rsrc((void*)bm, 0, (int)a.mtot_bytes, kRsrcWord3);
@@ -427,34 +436,41 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& a, int lblock, in
 #pragma unroll
         for (int f = 0; f < kNVar; ++f) yin.ldv(yin.base, f, vo, so, win[f][qz]);
     }
-    // halo rows of z = j0 and j0+1 straight into their levels
+    // edge row of z = j0 straight into its level; z = j0+1 into hreg (staged
+    // in the first step)
+    {
+        const unsigned so = zrow(j0) * siteb, s1 = zrow(j0 + 1) * siteb;
 #pragma unroll
-    for (int qz = 0; qz < 2; ++qz) {
-        const unsigned so = zrow(j0 + qz) * siteb;
+        for (int f = 0; f < kNVar; ++f) {
+            T x[V];
+            yin.ldv(yin.base, f, vh, so, x);
 #pragma unroll
-        for (int h = 0; h < NH; ++h)
+            for (int e = 0; e < V; ++e) xs[j0 & 1][f][slot_h + e] = x[e];
+        }
 #pragma unroll
-            for (int f = 0; f < kNVar; ++f) {
-                T x[V];
-                yin.ldv(yin.base, f, vh[h], so, x);
-#pragma unroll
-                for (int e = 0; e < V; ++e) xs[(j0 + qz) & 1][f][slot_h[h] + e] = x[e];
-            }
+        for (int f = 0; f < kNVar; ++f) yin.ldv(yin.base, f, vh, s1, hreg[f]);
     }
     double Ml = load_moment(j0);
 
     for (int j = j0; j < j1; ++j) {
         const unsigned so = (unsigned)(j + zoff) * siteb;
         const int lv = j & 1;
-        wave_sync();   // the previous iteration's LDS reads stay before these writes
-        // own rows of z = j into this level (the halo rows of z = j are there)
+        // own rows of z = j into this level (the edge rows of z = j are there)
 #pragma unroll
         for (int f = 0; f < kNVar; ++f)
 #pragma unroll
             for (int e = 0; e < V; ++e) xs[lv][f][slot_c + e] = win[f][2][e];
-        // ---- all loads of the iteration, in the order they are consumed:
-        // own row z+2 (stencil), u and w (update), halo rows z+2 (next
-        // iteration), moments of z+1 (next iteration)
+        // all rows of z = j are in level lv, and every wave is done with step
+        // j-1, whose level receives the edge rows of z = j+1 (loaded in step
+        // j-1; no wave reads that level before the next barrier)
+        block_sync();
+#pragma unroll
+        for (int f = 0; f < kNVar; ++f)
+#pragma unroll
+            for (int e = 0; e < V; ++e) xs[lv ^ 1][f][slot_h + e] = hreg[f][e];
+        // ---- the step's loads, in the order they are consumed: own row z+2
+        // (stencil), u and w (update), edge row z+2 (staged in the next
+        // step); the moments of z+1 are loaded at the end of the step
         const unsigned long long by = yin.fresh_base();
         const unsigned s2 = zrow(j + 2) * siteb;
 #pragma unroll
@@ -476,15 +492,8 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& a, int lblock, in
             __builtin_amdgcn_sched_barrier(0);
         }
 #pragma unroll
-        for (int h = 0; h < NH; ++h)
-#pragma unroll
-            for (int f = 0; f < kNVar; ++f) yin.ldv(by, f, vh[h], s2, hreg[h][f]);
+        for (int f = 0; f < kNVar; ++f) yin.ldv(by, f, vh, s2, hreg[f]);
         __builtin_amdgcn_sched_barrier(0);
-        // unconditional (a branch here would make every later wait a full
-        // drain); the last row of the box reloads itself
-        const double Mn = load_moment(min(j + 1, nz - 1));
-        __builtin_amdgcn_sched_barrier(0);
-        wave_sync();   // other lanes' writes are visible to the reads below
 
         T yo[kNVar][V], wn[kNVar][V];
 #pragma unroll
@@ -587,7 +596,7 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& a, int lblock, in
             asm volatile("" : "+s"(bp));
             const __amdgpu_buffer_rsrc_t rp =
                 __builtin_amdgcn_make_buffer_rsrc((void*)bp, 0, (int)a.mpart_bytes, kRsrcWord3);
-            const unsigned rec = ((unsigned)((row_ok ? r : nx) - i0) * (unsigned)nz * nch + (unsigned)ch) * (kMomRec * 8);
+            const unsigned rec = ((unsigned)((row_ok ? r : nx) - ib) * (unsigned)nz * nch + (unsigned)ch) * (kMomRec * 8);
             const unsigned srec = (mbase + j) * (unsigned)(nch * kMomRec * 8);
             Lane<double>::st(tot, rp, rec + (unsigned)(dl >> sh) * 8, srec);
         }
@@ -600,27 +609,20 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& a, int lblock, in
                 win[f][2][e] = win[f][3][e];
                 win[f][3][e] = win[f][4][e];
             }
-        Ml = Mn;
-        // halo rows of z = j+2 (loaded in this iteration) into this level,
-        // whose z = j reads are done; level lv ^ 1 holds z = j+1.  Done here
-        // rather than at the top of the next iteration: a wait at the loop
-        // head for a load of the previous iteration becomes a full drain of
-        // the memory counter.
-        wave_sync();
-#pragma unroll
-        for (int h = 0; h < NH; ++h)
-#pragma unroll
-            for (int f = 0; f < kNVar; ++f)
-#pragma unroll
-                for (int e = 0; e < V; ++e) xs[lv][f][slot_h[h] + e] = hreg[h][f][e];
+        // moments of z+1, issued last and into the variable the next step
+        // reads (so no register copy waits for it); unconditional (a branch
+        // here would make every later wait a full drain): the last row of the
+        // box reloads itself
+        __builtin_amdgcn_sched_barrier(0);
+        Ml = load_moment(min(j + 1, nz - 1));
     }
 }
 
 template <typename T, int R, int STAGE, bool COMP>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 2))) void k_stage(const StageArgs<T> a) {
-    __shared__ Slab<T, R> slab[4];
+    __shared__ Slab<T, R> slab;
     const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    stage_body<T, R, STAGE, COMP>(a, xcd_remap(blockIdx.x, gridDim.x), wib, slab[wib]);
+    stage_body<T, R, STAGE, COMP>(a, xcd_remap(blockIdx.x, gridDim.x), wib, slab);
 }
 
 // Many independent runs in one launch: logical block -> (run, block of run)
@@ -628,13 +630,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
 template <typename T, int R, int STAGE, bool COMP>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 2))) void k_stage_batch(
     const StageArgs<T>* __restrict__ runs, const int* __restrict__ bmap) {
-    __shared__ Slab<T, R> slab[4];
+    __shared__ Slab<T, R> slab;
     const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lb = xcd_remap(blockIdx.x, gridDim.x);
     const __attribute__((address_space(4))) int* cm = (const __attribute__((address_space(4))) int*)bmap;
     const int run = cm[2 * lb], blk = cm[2 * lb + 1];
     const StageArgs<T> a = runs[run];   // uniform, read-only: scalar loads
-    stage_body<T, R, STAGE, COMP>(a, blk, wib, slab[wib]);
+    stage_body<T, R, STAGE, COMP>(a, blk, wib, slab);
 }
 
 int rows_per_wave(int esize) { return esize == 8 ? StageShape<double>::R : StageShape<float>::R; }
@@ -677,7 +679,9 @@ StagePlan plan_stage_zlen(const Geometry& g, int zlen) {
 bool buffer_addressable(const Geometry& g, int esize) {
     const long long lim = (1ll << 32) - 1;
     const int R = rows_per_wave(esize);
-    const long long wave_rows = std::min<long long>(R + 5, g.nxs) * g.xrow * esize;
+    // the block's rows plus one: an idle lane's offset (the range's end) plus
+    // a scalar offset must not wrap
+    const long long wave_rows = (std::min<long long>(4 * R + 5, g.nxs) + 1) * g.xrow * esize;
     const long long part = (long long)(g.nx + 1) * g.nz * g.nch * kMomRec * (long long)sizeof(double);
     const long long tot = (long long)g.nx * g.nz * kMomStride * (long long)sizeof(double);
     return wave_rows < lim && part < lim && tot < lim;
