@@ -22,7 +22,7 @@
 //   Velocity moments: 12 doubles {A0..A3, B0..B3, C0..C3} per site with
 //   A = sum w u (bemr-emr), B = sum w u (bemi+emi), C = sum w u
 //   (bee-bmm-ee+mm), u = (1, vx, vy, vz).  The stage kernels write partial
-//   records [nx*nz sites][nch chunks][16] (12 used, each summed over the lpr
+//   records [nx*nz sites][nch chunks][12] (each summed over the lpr
 //   directions of one chunk) plus one spare x row of records; k_moment_totals
 //   adds them in chunk order into totals [nx*nz sites][16], which the next
 //   stage reads.
@@ -36,7 +36,7 @@ namespace nuosc {
 constexpr int kNVar = 8;
 constexpr int kWave = 64;
 constexpr int kMomStride = 16;   // doubles per per-site moment total record (12 used)
-constexpr int kMomRec = 16;      // doubles per (site, chunk) partial moment record (12 used)
+constexpr int kMomRec = 12;      // doubles per (site, chunk) partial moment record
 constexpr int kNMom = 12;
 constexpr int kNAna = 12;        // analysis accumulators (see AnaIdx)
 constexpr int kXg = 2;           // stored periodic ghost x rows at each end

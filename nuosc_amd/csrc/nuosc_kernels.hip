@@ -375,9 +375,11 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& a, int lblock, in
     const unsigned vo = (unsigned)min(own + kXg, spare) * xrowb + dof;
     // Stores are unconditional (no branch, so the compiler can count the
     // outstanding memory operations across iterations exactly): a lane past
-    // the last x row writes the spare row, a lane past nv writes its own
-    // padding columns; neither is ever read by a lane that stores a result.
-    const unsigned vos = (unsigned)(row_ok ? own + kXg : spare) * xrowb + (unsigned)(p * V) * sizeof(T);
+    // the last x row writes the spare row, which no lane that stores a result
+    // reads; a lane past nv works on the last direction (pl) throughout and
+    // stores the same bits to the same address as that direction's lane, so
+    // the padding columns are never written.
+    const unsigned vos = (unsigned)(row_ok ? own + kXg : spare) * xrowb + dof;
     // Edge rows: wave k loads block row -2, -1, 4R, 4R+1 (k = 0..3) with its
     // first L lanes.  The other lanes' offset lies past the descriptor's
     // range, so they fetch nothing (the load returns zeros), and they write
@@ -587,18 +589,21 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& a, int lblock, in
                     }
                 }
             }
-            // lane dl of a row holds the row total of value dl >> sh (12 values
-            // used; values 12..15 fill unused record slots; lanes sharing a
-            // value store the same bits)
+            // lane dl of a row holds the row total of value dl >> sh (values
+            // 12..15 are zeros, stored into the spare row's records, which
+            // nothing reads; lanes sharing a value store the same bits)
             const double tot = group_sum16<L, COMP>(t, te, lane);
             constexpr int sh = (L == 16) ? 0 : (L == 32 ? 1 : 2);
             unsigned long long bp = a.mpart;
             asm volatile("" : "+s"(bp));
             const __amdgpu_buffer_rsrc_t rp =
                 __builtin_amdgcn_make_buffer_rsrc((void*)bp, 0, (int)a.mpart_bytes, kRsrcWord3);
-            const unsigned rec = ((unsigned)((row_ok ? r : nx) - ib) * (unsigned)nz * nch + (unsigned)ch) * (kMomRec * 8);
+            const int sv = dl >> sh;                       // value index 0..15
+            const bool used = sv < kNMom;
+            const unsigned rec = ((unsigned)((row_ok && used ? r : nx) - ib) * (unsigned)nz * nch + (unsigned)ch) *
+                                 (kMomRec * 8);
             const unsigned srec = (mbase + j) * (unsigned)(nch * kMomRec * 8);
-            Lane<double>::st(tot, rp, rec + (unsigned)(dl >> sh) * 8, srec);
+            Lane<double>::st(tot, rp, rec + (unsigned)(used ? sv : sv - kNMom) * 8, srec);
         }
 #pragma unroll
         for (int f = 0; f < kNVar; ++f)
@@ -618,8 +623,16 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& a, int lblock, in
     }
 }
 
+// Waves per SIMD the register allocation aims at: the FP64 stages that read
+// a second state copy (2-4, 6, 7) need up to ~200 VGPRs and would spill at 3
+// waves; FP64 stages 0, 1, 5 fit 168 (3 waves), the FP32 ones 128 (4 waves).
+template <typename T, int STAGE>
+constexpr int stage_waves() {
+    return sizeof(T) == 4 ? 4 : (STAGE <= 1 || STAGE == 5 ? 3 : 2);
+}
+
 template <typename T, int R, int STAGE, bool COMP>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 2))) void k_stage(const StageArgs<T> a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(stage_waves<T, STAGE>()))) void k_stage(const StageArgs<T> a) {
     __shared__ Slab<T, R> slab;
     const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     stage_body<T, R, STAGE, COMP>(a, xcd_remap(blockIdx.x, gridDim.x), wib, slab);
@@ -628,7 +641,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
 // Many independent runs in one launch: logical block -> (run, block of run)
 // from bmap; each run's arguments from the runs table (read with scalar loads).
 template <typename T, int R, int STAGE, bool COMP>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 2))) void k_stage_batch(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(stage_waves<T, STAGE>()))) void k_stage_batch(
     const StageArgs<T>* __restrict__ runs, const int* __restrict__ bmap) {
     __shared__ Slab<T, R> slab;
     const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
