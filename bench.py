@@ -12,9 +12,12 @@ from a hipGraph) over the whole state, which stays resident in HBM.
 Prints one JSON line:
   value          site*dir RK4 updates/s = nx*nz*nv*K / wall(K steps)
   roofline       for the dominant kernel (RK4 stage 2): algorithmic bytes per
-                 launch (5 state passes x 8 fields x sizeof(real) per update)
+                 launch (4 state passes x 8 fields x sizeof(real) per update)
                  / its average duration from HIP events, against 8 TB/s
-  step_roofline  the same for a whole step (1024 B per update in FP64)
+  step_roofline  the same for a whole step (12 passes: 768 B per update in
+                 FP64), plus vs_16pass_ceiling = updates/s over 8 TB/s / 1024 B
+                 (FP64; 512 B FP32), the ceiling of the 16-pass RK4 streaming
+                 in which the per-update targets are stated
   cpu_baseline   the CPU oracle on a 128 x 128 x 642 crop of the same physics,
                  timed on this host's cores (a reported baseline)
 """
@@ -31,8 +34,9 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# state passes per update of each stage (read + write of the 8 fields)
-STAGE_PASSES = (3, 5, 5, 3)
+# state passes per update of each stage (read + write of the 8 fields):
+# S1 u -> Y2; S2 Y2, u -> Y3, A; S3 Y3, u -> Y4; S4 Y4, A -> u'
+STAGE_PASSES = (2, 4, 3, 3)
 
 
 def parse():
@@ -324,6 +328,7 @@ def main():
             "achieved": step_gbs,
             "frac": step_gbs / HBM_PEAK_GBS,
             "bytes_per_update": sum(STAGE_PASSES) * 8 * esize,
+            "vs_16pass_ceiling": value / (HBM_PEAK_GBS * 1e9 / (16 * 8 * esize)),
             "stage_ms": [round(x, 4) for x in stage_ms],
             "device_ms_per_step": dev_ms / a.steps,
         },

@@ -139,9 +139,9 @@ int ensure_tmp(nuosc_sim* s, size_t bytes) {
 template <typename T>
 int enqueue_step(nuosc_sim* s, hipEvent_t* ev = nullptr) {
     const double dt = s->dt;
-    const double c6 = 1.0 / 6.0 * dt, c3 = 1.0 / 3.0 * dt, c2 = 0.5 * dt;
+    const double c6 = 1.0 / 6.0 * dt, c2 = 0.5 * dt;
     T* u = static_cast<T*>(s->st_u);
-    T* w = static_cast<T*>(s->st_w);
+    T* w = static_cast<T*>(s->st_w);   // A = Y2 + u + dt k2 (see the stage kernel)
     T* ya = static_cast<T*>(s->st_ya);
     T* yb = static_cast<T*>(s->st_yb);
     const Geometry& g = s->geo;
@@ -162,17 +162,17 @@ int enqueue_step(nuosc_sim* s, hipEvent_t* ev = nullptr) {
     auto mark = [&](int k) { return ev ? hipEventRecord(ev[k], s->stream) : hipSuccess; };
     HIPCK(s, ghosts(u, 0.0));
     HIPCK(s, mark(0));
-    HIPCK(s, launch_stage<T>(1, g, s->ph, s->plan, u, nullptr, w, ya, s->mom_u, P, c6, c2, s->stream));
+    HIPCK(s, launch_stage<T>(1, g, s->ph, s->plan, u, nullptr, nullptr, ya, s->mom_u, P, 0.0, c2, s->stream));
     HIPCK(s, mark(1));
     HIPCK(s, launch_moment_totals(g, comp, P, s->mom_a, s->stream));
     HIPCK(s, ghosts(ya, c2));
     HIPCK(s, mark(2));
-    HIPCK(s, launch_stage<T>(2, g, s->ph, s->plan, ya, u, w, yb, s->mom_a, P, c3, c2, s->stream));
+    HIPCK(s, launch_stage<T>(2, g, s->ph, s->plan, ya, u, w, yb, s->mom_a, P, dt, c2, s->stream));
     HIPCK(s, mark(3));
     HIPCK(s, launch_moment_totals(g, comp, P, s->mom_b, s->stream));
     HIPCK(s, ghosts(yb, c2));
     HIPCK(s, mark(4));
-    HIPCK(s, launch_stage<T>(3, g, s->ph, s->plan, yb, u, w, ya, s->mom_b, P, c3, dt, s->stream));
+    HIPCK(s, launch_stage<T>(3, g, s->ph, s->plan, yb, u, nullptr, ya, s->mom_b, P, 0.0, dt, s->stream));
     HIPCK(s, mark(5));
     HIPCK(s, launch_moment_totals(g, comp, P, s->mom_a, s->stream));
     HIPCK(s, ghosts(ya, dt));
@@ -281,10 +281,10 @@ int check_field(nuosc_sim* s, int f) {
 }
 
 // RK4 stages 1-3 with their boundary fills and moment totals: they write only
-// the scratch copies w, yA, yB and the scratch moment records, never u.
+// the scratch copies A (w), yA, yB and the scratch moment records, never u.
 template <typename T>
 hipError_t enqueue_scratch_stages(nuosc_sim* s) {
-    const double dt = s->dt, c6 = dt / 6, c3 = dt / 3, c2 = dt / 2;
+    const double dt = s->dt, c2 = dt / 2;
     const Geometry& g = s->geo;
     T* u = static_cast<T*>(s->st_u);
     T* w = static_cast<T*>(s->st_w);
@@ -300,13 +300,13 @@ hipError_t enqueue_scratch_stages(nuosc_sim* s) {
     };
     auto run = [&](hipError_t r) { if (e == hipSuccess) e = r; };
     ghosts(u, 0.0);
-    run(launch_stage<T>(1, g, s->ph, s->plan, u, nullptr, w, ya, s->mom_u, P, c6, c2, s->stream));
+    run(launch_stage<T>(1, g, s->ph, s->plan, u, nullptr, nullptr, ya, s->mom_u, P, 0.0, c2, s->stream));
     run(launch_moment_totals(g, comp, P, s->mom_a, s->stream));
     ghosts(ya, c2);
-    run(launch_stage<T>(2, g, s->ph, s->plan, ya, u, w, yb, s->mom_a, P, c3, c2, s->stream));
+    run(launch_stage<T>(2, g, s->ph, s->plan, ya, u, w, yb, s->mom_a, P, dt, c2, s->stream));
     run(launch_moment_totals(g, comp, P, s->mom_b, s->stream));
     ghosts(yb, c2);
-    run(launch_stage<T>(3, g, s->ph, s->plan, yb, u, w, ya, s->mom_b, P, c3, dt, s->stream));
+    run(launch_stage<T>(3, g, s->ph, s->plan, yb, u, nullptr, ya, s->mom_b, P, 0.0, dt, s->stream));
     return e;
 }
 }  // namespace

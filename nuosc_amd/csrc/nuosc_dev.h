@@ -84,11 +84,14 @@ struct Physics {
     double mu, pmo, ct, st;
 };
 
-// One RK4 stage over the whole grid.
-//   stage 0: yout = L(yin)                       (RHS only; test primitive)
-//   stage 1: w = yin + cw*k,  yout = yin + cy*k   (yin is the state u)
-//   stage 2/3: w = w + cw*k,  yout = u + cy*k
-//   stage 4: yout = w + cw*k                      (yout is the new state)
+// One RK4 stage over the whole grid (k = L(yin)).
+//   stage 0: yout = k                              (RHS only; test primitive)
+//   stage 1: yout = yin + cy*k                     (yin is the state u)
+//   stage 2: w = (yin + u) + cw*k,  yout = u + cy*k
+//   stage 3: yout = u + cy*k
+//   stage 4: yout = (w + yin)/3 + cw*k            (yout is the new state)
+// With cy = dt/2, (dt, dt/2), dt and cw = dt/6 this is classic RK4 in 12
+// passes over the state (w holds A = Y2 + 2 Y3 - u).
 // SSP-RK3 (k = L(yin)):
 //   stage 5: yout = yin + cy*k
 //   stage 6/7: yout = ca*u + cb*(yin + cy*k)        (stage 7 may write u in place)

@@ -14,14 +14,19 @@
 // per-(site, chunk) partial sums, so no separate pass over the state is
 // needed; the consumer adds the partials of a site in chunk order.
 //
-// RK4 is streamed with four state copies u, w, yA, yB (w accumulates the
-// final combination), i.e. 16 passes over the 8 fields per step:
-//   S1: read u            -> w = u + dt/6 k1,  yA = u + dt/2 k1
-//   S2: read yA, u, w     -> w += dt/3 k2,     yB = u + dt/2 k2
-//   S3: read yB, u, w     -> w += dt/3 k3,     yA = u + dt k3
-//   S4: read yA, w        -> u  = w + dt/6 k4
+// RK4 is streamed with four state copies u, A, yA, yB in 12 passes over the
+// 8 fields per step.  With the stage inputs Y2 = u + dt/2 k1,
+// Y3 = u + dt/2 k2, Y4 = u + dt k3, the new state is
+//   u + dt/6 (k1 + 2k2 + 2k3 + k4) = (A + Y4) / 3 + dt/6 k4,
+//   A = Y2 + 2 Y3 - u = Y2 + u + dt k2,
+// so no running sum of the k's has to be read and rewritten every stage:
+//   S1: read u            -> yA = Y2
+//   S2: read yA, u        -> yB = Y3,  A = Y2 + u + dt k2
+//   S3: read yB, u        -> yA = Y4
+//   S4: read yA, A        -> u  = (A + Y4) / 3 + dt/6 k4
 // which equals the reference's combination u + dt/6 (k4 + (k1 + 2k2) + 2k3)
-// (nuosc_class.cpp:374-427) up to rounding.
+// (nuosc_class.cpp:374-427) up to rounding (A ~ 2u and Y4 ~ u, so no
+// cancellation).
 //
 // The right-hand side follows nuosc_class.cpp:11-307 (2 flavours, 3rd-order
 // Kreiss-Oliger, 4th-order central advection, vacuum term), with the
@@ -411,6 +416,7 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& a, int lblock, in
     const T mu2 = (T)a.mu2, mu = (T)a.mu, p2st = (T)a.p2st, p2ct = (T)a.p2ct;
     const T pmo = (T)a.pmo, ct2 = (T)a.ct2, st = (T)a.st;
     const T cw = (T)a.cw, cy = (T)a.cy, ca = (T)a.ca, cb = (T)a.cb;
+    const T third = (T)(1.0 / 3.0);
     // moment totals of the lane's site (a lane past the last row reads row
     // nx-1): lane dl holds value dl % 16 of its row's record; every data row
     // (16 or 32 lanes) is made of whole 16-lane DPP rows, so row_project()
@@ -487,7 +493,7 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& a, int lblock, in
             for (int f = 0; f < kNVar; ++f) uu.template ldv<2>(bu, f, vo, so, uc[f]);
             __builtin_amdgcn_sched_barrier(0);
         }
-        if constexpr (STAGE >= 2 && STAGE <= 4) {
+        if constexpr (STAGE == 4) {
             const unsigned long long bw = ww.fresh_base();
 #pragma unroll
             for (int f = 0; f < kNVar; ++f) ww.template ldv<2>(bw, f, vo, so, wc[f]);
@@ -534,14 +540,15 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& a, int lblock, in
                 const T k = loc[f] + adv + ko + vac[f];
                 if constexpr (STAGE == 0) {
                     yo[f][e] = k;
-                } else if constexpr (STAGE == 1) {
-                    wn[f][e] = c0 + cw * k;
+                } else if constexpr (STAGE == 1) {   // Y2 = u + dt/2 k1
                     yo[f][e] = c0 + cy * k;
-                } else if constexpr (STAGE == 2 || STAGE == 3) {
-                    wn[f][e] = wc[f][e] + cw * k;
+                } else if constexpr (STAGE == 2) {   // A = Y2 + u + dt k2, Y3 = u + dt/2 k2
+                    wn[f][e] = (c0 + uc[f][e]) + cw * k;
                     yo[f][e] = uc[f][e] + cy * k;
-                } else if constexpr (STAGE == 4) {
-                    yo[f][e] = wc[f][e] + cw * k;
+                } else if constexpr (STAGE == 3) {   // Y4 = u + dt k3
+                    yo[f][e] = uc[f][e] + cy * k;
+                } else if constexpr (STAGE == 4) {   // u' = (A + Y4) / 3 + dt/6 k4
+                    yo[f][e] = (wc[f][e] + c0) * third + cw * k;
                 } else if constexpr (STAGE == 5) {   // SSP-RK3 (nuosc_class.cpp:436)
                     yo[f][e] = c0 + cy * k;
                 } else {                             // SSP-RK3 (nuosc_class.cpp:440, 444, 369)
@@ -554,7 +561,7 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& a, int lblock, in
             const unsigned long long bo = yout.fresh_base();
 #pragma unroll
             for (int f = 0; f < kNVar; ++f) {
-                if constexpr (STAGE >= 1 && STAGE <= 3) ww.stv(bw, f, wn[f], vos, so);
+                if constexpr (STAGE == 2) ww.stv(bw, f, wn[f], vos, so);
                 yout.stv(bo, f, yo[f], vos, so);
             }
         }
@@ -623,12 +630,12 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& a, int lblock, in
     }
 }
 
-// Waves per SIMD the register allocation aims at: the FP64 stages that read
-// a second state copy (2-4, 6, 7) need up to ~200 VGPRs and would spill at 3
-// waves; FP64 stages 0, 1, 5 fit 168 (3 waves), the FP32 ones 128 (4 waves).
+// Waves per SIMD the register allocation aims at: FP64 stages 0-5 fit 168
+// VGPRs (3 waves), the SSP-RK3 stages 6 and 7 would spill there (2 waves);
+// the FP32 stages fit 128 (4 waves).
 template <typename T, int STAGE>
 constexpr int stage_waves() {
-    return sizeof(T) == 4 ? 4 : (STAGE <= 1 || STAGE == 5 ? 3 : 2);
+    return sizeof(T) == 4 ? 4 : (STAGE <= 5 ? 3 : 2);
 }
 
 template <typename T, int R, int STAGE, bool COMP>
@@ -916,7 +923,7 @@ static hipError_t batch_build_t(BatchTables& t, const std::vector<BatchMember>& 
         std::vector<XgArgs> xgs(n);
         for (int r = 0; r < n; ++r) {
             const BatchMember& m = ms[r];
-            const double dt = m.dt, c6 = 1.0 / 6.0 * dt, c3 = 1.0 / 3.0 * dt, c2 = 0.5 * dt;
+            const double dt = m.dt, c6 = 1.0 / 6.0 * dt, c2 = 0.5 * dt;
             T *u = (T*)m.u, *w = (T*)m.w, *ya = (T*)m.ya, *yb = (T*)m.yb;
             T* yin;
             const T* uu;
@@ -925,9 +932,9 @@ static hipError_t batch_build_t(BatchTables& t, const std::vector<BatchMember>& 
             double* mo;
             double cw, cy;
             switch (st) {
-                case 1: yin = u; uu = u; yout = ya; mt = m.mu; mo = m.ma; cw = c6; cy = c2; break;
-                case 2: yin = ya; uu = u; yout = yb; mt = m.ma; mo = m.mb; cw = c3; cy = c2; break;
-                case 3: yin = yb; uu = u; yout = ya; mt = m.mb; mo = m.ma; cw = c3; cy = dt; break;
+                case 1: yin = u; uu = u; yout = ya; mt = m.mu; mo = m.ma; cw = 0.0; cy = c2; break;
+                case 2: yin = ya; uu = u; yout = yb; mt = m.ma; mo = m.mb; cw = dt; cy = c2; break;
+                case 3: yin = yb; uu = u; yout = ya; mt = m.mb; mo = m.ma; cw = 0.0; cy = dt; break;
                 default: yin = ya; uu = ya; yout = u; mt = m.ma; mo = m.mu; cw = c6; cy = 0.0; break;
             }
             args[r] = make_stage_args<T>(m.g, m.ph, m.pl, yin, uu, w, yout, mt, m.mpart, cw, cy);
