@@ -442,6 +442,7 @@ int nuosc_create(const nuosc_config* cfg, nuosc_sim** out) {
     s->geo.nch = s->nch;
     s->geo.lpr = lpr;
     s->geo.rows = rows_per_wave((int)s->esize);
+    s->geo.wpb = waves_per_block((int)s->esize);
     s->geo.nxs = s->nx + 2 * kXg + 1;
     s->geo.srec = s->srec;
     s->geo.xrow = s->xrow;

@@ -50,6 +50,7 @@ struct Geometry {
     int nx, nz, nv, nvp;
     int lpr;               // directions per chunk: one 128-byte row segment (dirs_per_chunk)
     int rows;              // x rows per stage-kernel wavefront (rows_per_wave)
+    int wpb;               // wavefronts per stage-kernel block (waves_per_block)
     int nch;               // direction chunks per site = ceil(nv / lpr)
     int nxs;               // stored x rows: nx + 2 kXg + 1 (ghost rows and the spare row)
     long long srec;        // elements between consecutive site records of one copy (8 nvp)
@@ -106,6 +107,8 @@ struct StagePlan {
 int rows_per_wave(int esize);
 // directions per chunk = one 128-byte row segment (FP64 16, FP32 32)
 int dirs_per_chunk(int esize);
+// wavefronts per stage-kernel block (4)
+int waves_per_block(int esize);
 
 StagePlan plan_stage(const Geometry& g, int num_cus);
 // the same decomposition with a given z-segment length

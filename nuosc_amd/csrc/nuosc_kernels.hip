@@ -47,13 +47,17 @@ namespace nuosc {
 // 4 x 16 lanes x 2: it keeps 3 waves per SIMD).
 template <typename T>
 struct StageShape;
+// W waves per block: a block covers W R adjacent x rows of one chunk and
+// loads the 4 rows beyond its edges as halos.  4 waves in both precisions:
+// FP32 blocks of 8 waves (16 rows) read fewer edge rows but measured slower
+// (the per-step barrier waits for more waves).
 template <>
 struct StageShape<double> {
-    static constexpr int R = 4, V = 1;
+    static constexpr int R = 4, V = 1, W = 4;
 };
 template <>
 struct StageShape<float> {
-    static constexpr int R = 2, V = 1;
+    static constexpr int R = 2, V = 1, W = 4;
 };
 template <typename T>
 constexpr int kVals = StageShape<T>::V;
@@ -305,12 +309,12 @@ __device__ __forceinline__ double group_sum16(const double (&a)[16], const doubl
     }
 }
 
-// x rows staged per block: block rows -2 .. 4R+1 (slots 0 .. 4R+3) and one
-// slot that idle lanes write; D = L V directions each
-template <typename T, int R>
+// x rows staged per block of W waves: block rows -2 .. WR+1 (slots
+// 0 .. WR+3) and one slot that idle lanes write; D = L V directions each
+template <typename T, int R, int W>
 struct Slab {
     static constexpr int D = (kWave / R) * kVals<T>;
-    static constexpr int NS = 4 * R + 5;
+    static constexpr int NS = W * R + 5;
     T v[2][kNVar][NS * D];   // two z levels (by parity) of all 8 fields
 };
 
@@ -323,12 +327,12 @@ __device__ __forceinline__ void block_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-// One wavefront's work.  Block lblock = (group of 4 x-row quads, direction
+// One wavefront's work.  Block lblock = (group of W x-row quads, direction
 // chunk, z segment), quad group fastest; wave wib of the block takes quad
-// 4 * group + wib.  Lane (g, dl) of a wave handles x row i0 + g and the V
+// W * group + wib.  Lane (g, dl) of a wave handles x row i0 + g and the V
 // adjacent directions ch * D + dl * V + e, e < V.
-template <typename T, int R, int STAGE, bool COMP>
-__device__ __forceinline__ void stage_body(const StageArgs<T>& a, int lblock, int wib, Slab<T, R>& slab) {
+template <typename T, int R, int W, int STAGE, bool COMP>
+__device__ __forceinline__ void stage_body(const StageArgs<T>& a, int lblock, int wib, Slab<T, R, W>& slab) {
     constexpr int V = kVals<T>;    // directions per lane (one 8-byte load)
     constexpr int L = kWave / R;   // lanes per row
     constexpr int D = L * V;       // directions per row chunk (one 128-byte line)
@@ -347,8 +351,8 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& a, int lblock, in
     // and starts at about the same time, own the block's edge halo rows.
     // (Chunk fastest, which keeps the chunks of a site record together,
     // measured 3-7% slower.)
-    const int nqg = (a.nquads + 3) >> 2;
-    const int ib = (lblock % nqg) * 4 * R;   // first x row of the block
+    const int nqg = (a.nquads + W - 1) / W;
+    const int ib = (lblock % nqg) * W * R;   // first x row of the block
     const int rest = lblock / nqg;
     const int ch = rest % nch;
     const int zs = rest / nch;
@@ -369,7 +373,7 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& a, int lblock, in
     // lane's stored row relative to ib goes into its vector offset.
     const int spare = nx + 2 * kXg - ib;     // spare stored row, relative to ib
     const unsigned long long woff = (unsigned long long)ib * xrowb;
-    const unsigned wbytes = (unsigned)(min(4 * R + 5, a.g.nxs - ib)) * xrowb;
+    const unsigned wbytes = (unsigned)(min(W * R + 5, a.g.nxs - ib)) * xrowb;
     const Rows<T> yin{a.yin + woff, wbytes, rowb}, uu{a.u + woff, wbytes, rowb};
     const Rows<T> ww{a.w + woff, wbytes, rowb}, yout{a.yout + woff, wbytes, rowb};
     // stored z row of interior z index jj (periodic: wrapped; open: the ghost
@@ -385,14 +389,14 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& a, int lblock, in
     // stores the same bits to the same address as that direction's lane, so
     // the padding columns are never written.
     const unsigned vos = (unsigned)(row_ok ? own + kXg : spare) * xrowb + dof;
-    // Edge rows: wave k loads block row -2, -1, 4R, 4R+1 (k = 0..3) with its
-    // first L lanes.  The other lanes' offset lies past the descriptor's
+    // Edge rows: wave k < 4 loads block row -2, -1, WR, WR+1 (k = 0..3) with
+    // its first L lanes.  The other lanes' offset lies past the descriptor's
     // range, so they fetch nothing (the load returns zeros), and they write
     // the idle slot.
-    const bool eln = g == 0;
-    const int erow = wib < 2 ? wib : min(4 * R + wib, spare);   // stored row relative to ib
+    const bool eln = g == 0 && wib < 4;
+    const int erow = wib < 2 ? wib : min(W * R + wib, spare);   // stored row relative to ib
     const unsigned vh = eln ? (unsigned)erow * xrowb + dof : wbytes;
-    const int slot_h = (eln ? (wib < 2 ? wib : 4 * R + wib) : 4 * R + 4) * D + dl * V;
+    const int slot_h = (eln ? (wib < 2 ? wib : W * R + wib) : W * R + 4) * D + dl * V;
     const int slot_c = (own + 2) * D + dl * V;
 
     // per-direction constants (nuosc_class.cpp:119-121, 180-181)
@@ -639,27 +643,30 @@ constexpr int stage_waves() {
 }
 
 template <typename T, int R, int STAGE, bool COMP>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(stage_waves<T, STAGE>()))) void k_stage(const StageArgs<T> a) {
-    __shared__ Slab<T, R> slab;
+__global__ __launch_bounds__(64 * StageShape<T>::W) __attribute__((amdgpu_waves_per_eu(stage_waves<T, STAGE>()))) void k_stage(const StageArgs<T> a) {
+    constexpr int W = StageShape<T>::W;
+    __shared__ Slab<T, R, W> slab;
     const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    stage_body<T, R, STAGE, COMP>(a, xcd_remap(blockIdx.x, gridDim.x), wib, slab);
+    stage_body<T, R, W, STAGE, COMP>(a, xcd_remap(blockIdx.x, gridDim.x), wib, slab);
 }
 
 // Many independent runs in one launch: logical block -> (run, block of run)
 // from bmap; each run's arguments from the runs table (read with scalar loads).
 template <typename T, int R, int STAGE, bool COMP>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(stage_waves<T, STAGE>()))) void k_stage_batch(
+__global__ __launch_bounds__(64 * StageShape<T>::W) __attribute__((amdgpu_waves_per_eu(stage_waves<T, STAGE>()))) void k_stage_batch(
     const StageArgs<T>* __restrict__ runs, const int* __restrict__ bmap) {
-    __shared__ Slab<T, R> slab;
+    constexpr int W = StageShape<T>::W;
+    __shared__ Slab<T, R, W> slab;
     const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lb = xcd_remap(blockIdx.x, gridDim.x);
     const __attribute__((address_space(4))) int* cm = (const __attribute__((address_space(4))) int*)bmap;
     const int run = cm[2 * lb], blk = cm[2 * lb + 1];
     const StageArgs<T> a = runs[run];   // uniform, read-only: scalar loads
-    stage_body<T, R, STAGE, COMP>(a, blk, wib, slab);
+    stage_body<T, R, W, STAGE, COMP>(a, blk, wib, slab);
 }
 
 int rows_per_wave(int esize) { return esize == 8 ? StageShape<double>::R : StageShape<float>::R; }
+int waves_per_block(int esize) { return esize == 8 ? StageShape<double>::W : StageShape<float>::W; }
 int dirs_per_chunk(int esize) {
     return esize == 8 ? (kWave / StageShape<double>::R) * StageShape<double>::V
                       : (kWave / StageShape<float>::R) * StageShape<float>::V;
@@ -669,7 +676,7 @@ StagePlan plan_stage(const Geometry& g, int num_cus) {
     // enough waves to give every SIMD several (>= 12 per CU), segments long
     // enough that the window warm-up stays a small overhead
     StagePlan p{};
-    const int R = g.rows;
+    const int R = g.rows, W = g.wpb;
     const long long quads = (g.nx + R - 1) / R;
     const long long rows = quads * g.nch;
     const long long want = (long long)num_cus * 48;
@@ -679,29 +686,29 @@ StagePlan plan_stage(const Geometry& g, int num_cus) {
     p.zlen = zlen;
     p.nseg = (g.nz + zlen - 1) / zlen;
     p.nquads = (int)quads;
-    p.nblocks = (int)(((quads + 3) / 4) * g.nch * p.nseg);
-    p.ntask = p.nblocks * 4;
+    p.nblocks = (int)(((quads + W - 1) / W) * g.nch * p.nseg);
+    p.ntask = p.nblocks * W;
     return p;
 }
 
 StagePlan plan_stage_zlen(const Geometry& g, int zlen) {
     StagePlan p{};
-    const int R = g.rows;
+    const int R = g.rows, W = g.wpb;
     const long long quads = (g.nx + R - 1) / R;
     p.zlen = std::max(1, std::min(zlen, g.nz));
     p.nseg = (g.nz + p.zlen - 1) / p.zlen;
     p.nquads = (int)quads;
-    p.nblocks = (int)(((quads + 3) / 4) * g.nch * p.nseg);
-    p.ntask = p.nblocks * 4;
+    p.nblocks = (int)(((quads + W - 1) / W) * g.nch * p.nseg);
+    p.ntask = p.nblocks * W;
     return p;
 }
 
 bool buffer_addressable(const Geometry& g, int esize) {
     const long long lim = (1ll << 32) - 1;
-    const int R = rows_per_wave(esize);
+    const int R = rows_per_wave(esize), W = waves_per_block(esize);
     // the block's rows plus one: an idle lane's offset (the range's end) plus
     // a scalar offset must not wrap
-    const long long wave_rows = (std::min<long long>(4 * R + 5, g.nxs) + 1) * g.xrow * esize;
+    const long long wave_rows = (std::min<long long>(W * R + 5, g.nxs) + 1) * g.xrow * esize;
     const long long part = (long long)(g.nx + 1) * g.nz * g.nch * kMomRec * (long long)sizeof(double);
     const long long tot = (long long)g.nx * g.nz * kMomStride * (long long)sizeof(double);
     return wave_rows < lim && part < lim && tot < lim;
@@ -745,8 +752,8 @@ hipError_t launch_stage(int stage, const Geometry& g, const Physics& ph, const S
     const StageArgs<T> a = make_stage_args<T>(g, ph, pl, yin, u, w, yout, mtot, mpart, cw, cy, ca, cb);
     constexpr bool COMP = sizeof(T) == 4;
     constexpr int R = StageShape<T>::R;
-    if (g.lpr != dirs_per_chunk((int)sizeof(T))) return hipErrorInvalidValue;
-    dim3 grid(pl.nblocks), block(256);
+    if (g.lpr != dirs_per_chunk((int)sizeof(T)) || g.wpb != StageShape<T>::W) return hipErrorInvalidValue;
+    dim3 grid(pl.nblocks), block(64 * StageShape<T>::W);
     switch (stage) {
         case 0: hipLaunchKernelGGL((k_stage<T, R, 0, COMP>), grid, block, 0, s, a); break;
         case 1: hipLaunchKernelGGL((k_stage<T, R, 1, COMP>), grid, block, 0, s, a); break;
@@ -963,7 +970,7 @@ hipError_t batch_build(BatchTables& t, const std::vector<BatchMember>& ms, int e
     if (ms.empty()) return hipSuccess;
     for (const auto& m : ms) {
         if (!buffer_addressable(m.g, esize)) return hipErrorInvalidValue;
-        if (m.g.lpr != dirs_per_chunk(esize)) return hipErrorInvalidValue;
+        if (m.g.lpr != dirs_per_chunk(esize) || m.g.wpb != waves_per_block(esize)) return hipErrorInvalidValue;
     }
     return esize == 8 ? batch_build_t<double>(t, ms, s) : batch_build_t<float>(t, ms, s);
 }
@@ -986,7 +993,7 @@ template <typename T>
 static hipError_t batch_stage_t(const BatchTables& t, int st, hipStream_t s) {
     constexpr int R = StageShape<T>::R;
     constexpr bool COMP = sizeof(T) == 4;
-    const dim3 grid(t.nblocks), block(256);
+    const dim3 grid(t.nblocks), block(64 * StageShape<T>::W);
     const StageArgs<T>* a = (const StageArgs<T>*)t.args[st - 1];
     switch (st) {
         case 1: hipLaunchKernelGGL((k_stage_batch<T, R, 1, COMP>), grid, block, 0, s, a, t.bmap); break;
