@@ -51,9 +51,13 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-autotune", action="store_true", help="box: keep the default z-segment plan")
     p.add_argument("--cpu-seconds", type=float, default=12.0)
-    p.add_argument("--workload", default="box", choices=("box", "batched", "served"),
+    p.add_argument("--workload", default="box", choices=("box", "batched", "served", "precision"),
                    help="box: one large run (default); batched: 64 uneven runs stepped together vs one by one; "
-                        "served: Poisson arrivals into the run server")
+                        "served: Poisson arrivals into the run server; precision: the FP32 form against FP64 "
+                        "from the same initial state (--shape box|column)")
+    p.add_argument("--shape", default="box", choices=("box", "column"),
+                   help="precision: the 512x512x642 periodic box (seeded eps=1e-6) or the 1-D open column "
+                        "(4096 cells in z, 642 directions, gz=3, an ipt-0 pulse entering through z0)")
     p.add_argument("--runs", type=int, default=64)
     p.add_argument("--grids", default=None,
                    help="batched/served: comma-separated square grid sizes the runs are drawn from "
@@ -219,6 +223,83 @@ def bench_served(a):
     print(json.dumps(out), flush=True)
 
 
+def bench_precision(a):
+    """The FP32 form against FP64 from the same initial state: every ANA_EVERY
+    = 10 steps the Pee and |rho_ex| series of both, at the end max|dfield| per
+    field, and the rates of change of log|rho_ex| of both."""
+    import numpy as np
+
+    import nuosc_amd as N
+    if a.shape == "box":
+        half = a.zmax
+        geo = dict(nv=a.nn, x0=-half, x1=half, z0=-half, z1=half, dx=a.dz, dz=a.dz, CFL=0.4, ko=0.1, mu=1.0,
+                   pmo=0.0)
+        init = dict(ipt=5, alpha=0.9, eps0=1e-6, sigma=1.0, lnue=0.6, lnueb=0.5, seed=12345)
+        label = "512x512 periodic box x 642 directions, seeded eps=1e-6"
+    else:
+        geo = dict(nv=a.nn, x0=-0.025, x1=0.025, z0=-102.4, z1=102.4, dx=0.05, dz=0.05, CFL=0.4, ko=0.1, mu=1.0,
+                   pmo=0.0, gz=3, bc=N.BC_OPEN_Z, zc=-105.4)
+        init = dict(ipt=0, alpha=0.9, eps0=1e-6, sigma=1.0, lnue=0.6, lnueb=0.5)
+        label = "1-D open column, 4096 cells x 642 directions, gz=3, ipt-0 pulse entering through z0"
+    sims = {}
+    for prec in (64, 32):
+        g = N.NuOsc(device=0, precision=N.PREC_FP64 if prec == 64 else N.PREC_FP32, use_graph=True, **geo)
+        g.fillInitValue(**init)
+        sims[prec] = g
+    ana_every = 10
+    series = {64: [], 32: []}
+    for step in range(0, a.steps + 1, ana_every):
+        for prec, g in sims.items():
+            if step:
+                g.step_rk4(ana_every)
+            r = g.analysis()
+            series[prec].append((r.phy_time, r.surv, r.maxdP, np.sqrt(max(r.rho_ex2, 0.0))))
+        if step % 100 == 0:
+            print(f"precision: step {step}/{a.steps}", file=sys.stderr, flush=True)
+    s64, s32 = np.array(series[64]), np.array(series[32])
+    dfield = []
+    for f in range(8):
+        f64, f32 = sims[64].get_field(f), sims[32].get_field(f)
+        dfield.append(float(np.max(np.abs(f32 - f64)) / max(np.max(np.abs(f64)), 1e-300)))
+        del f64, f32
+    t, r64, r32 = s64[:, 0], s64[:, 3], s32[:, 3]
+    # mean rate of change of log|rho_ex| over the run, and the fitted rate
+    # over the longest stretch in which the FP64 amplitude keeps growing by
+    # at least 1% per sample (a linear-growth window), if there is one
+    mean64 = float((np.log(r64[-1]) - np.log(r64[0])) / (t[-1] - t[0]))
+    mean32 = float((np.log(r32[-1]) - np.log(r32[0])) / (t[-1] - t[0]))
+    grow = np.concatenate([[False], r64[1:] > 1.01 * r64[:-1]])
+    best, cur, start = (0, 0), 0, 0
+    for k, gk in enumerate(grow):
+        cur = cur + 1 if gk else 0
+        if cur == 1:
+            start = k - 1
+        if cur > best[1] - best[0]:
+            best = (start, k + 1)
+    growth = {"mean_rate_fp64": mean64, "mean_rate_fp32": mean32,
+              "mean_rate_rel_error": abs(mean32 - mean64) / max(abs(mean64), 1e-300)}
+    if best[1] - best[0] >= 6:
+        sl = slice(best[0], best[1])
+        g64 = float(np.polyfit(t[sl], np.log(r64[sl]), 1)[0])
+        g32 = float(np.polyfit(t[sl], np.log(r32[sl]), 1)[0])
+        growth.update({"window_rate_fp64": g64, "window_rate_fp32": g32,
+                       "window_rate_rel_error": abs(g32 - g64) / abs(g64),
+                       "window_t": [float(t[sl][0]), float(t[sl][-1])]})
+    for g in sims.values():
+        g.close()
+    out = {
+        "metric": "FP32 form vs FP64, same initial state",
+        "steps": a.steps,
+        "config": {"workload": label},
+        "max_rel_dfield": dfield,
+        "pee_max_abs_error": float(np.max(np.abs(s32[:, 1] - s64[:, 1]))),
+        "max_abs_P_minus_1": {"fp64": float(np.max(s64[:, 2])), "fp32": float(np.max(s32[:, 2]))},
+        "rho_ex": {"initial": float(r64[0]), "final_fp64": float(r64[-1]), "final_fp32": float(r32[-1])},
+        "growth_rate": growth,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def cpu_baseline(nn, dz, seconds):
     """Oracle (OpenMP C restatement of the reference) on a 128x128 crop."""
     from oracle import oracle as O
@@ -253,17 +334,20 @@ def main():
     if a.copy_calibration:
         copy_calibration()
         return
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {a.gpus}); "
+              "the periodic box would shard as an (x,z) domain decomposition with radius-2 halos.",
+              file=sys.stderr)
+        sys.exit(2)
     if a.workload == "batched":
         bench_batched(a)
         return
     if a.workload == "served":
         bench_served(a)
         return
-    if a.gpus != 1:
-        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {a.gpus}); "
-              "the periodic box would shard as an (x,z) domain decomposition with radius-2 halos.",
-              file=sys.stderr)
-        sys.exit(2)
+    if a.workload == "precision":
+        bench_precision(a)
+        return
     import nuosc_amd as N
 
     prec = N.PREC_FP64 if a.precision == 64 else N.PREC_FP32
