@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define NUOSC_ABI_VERSION 2
+#define NUOSC_ABI_VERSION 3
 
 /* status codes */
 #define NUOSC_OK 0
@@ -61,6 +61,21 @@ extern "C" {
                              * quadratic extrapolation of the three edge cells.  Needs
                              * nz >= 3 and gz in {2, 3}; not batchable. */
 
+/* Discretisation of calRHS: the reference's compile-time variants
+ * (nuosc_class.cpp:112-183, selected there by KO_ORD_3 and ADVEC_*), chosen
+ * per handle here.  Kreiss-Oliger order 3 (KO_ORD_3, the default: -ko/(16h)
+ * delta^4) or 5 (-ko/(64h) delta^6, as written in the reference, whose sign
+ * makes it amplify rather than damp the highest wavenumbers); advection: */
+#define NUOSC_ADV_CFD4 0      /* 4th-order central (default) */
+#define NUOSC_ADV_CFD2 1      /* ADVEC_CFD2O: 2nd-order central */
+#define NUOSC_ADV_UPWIND2 2   /* ADVEC_UPWIND2O: 2nd-order upwind */
+#define NUOSC_ADV_LOPSIDED 3  /* ADVEC_LOPESIDE: 4th-order lopsided */
+#define NUOSC_ADV_MUSCL 4     /* ADVEC_MUSCL: flux limited (superbee) */
+#define NUOSC_ADV_OFF 5       /* ADVEC_OFF */
+/* KO order 5 and the lopsided scheme have stencil radius 3: with the open z
+ * boundary they need gz = 3.  Handles with a non-default discretisation are
+ * not batchable. */
+
 typedef struct nuosc_sim nuosc_sim;
 
 /* Run configuration.  Replaces the NuOsc constructor arguments
@@ -84,6 +99,8 @@ typedef struct nuosc_config {
     int use_graph;   /* capture one step as a hipGraph and replay it */
     double zc;       /* centre of the ipt-0 profile eps0 exp(-(z-zc)^2/(2 sigma^2));
                       * the reference's is 0 (nuosc_init.cpp:4, 98) */
+    int ko_order;    /* Kreiss-Oliger order: 3 (or 0: the default) or 5 */
+    int adv;         /* NUOSC_ADV_* (0: the default central scheme) */
 } nuosc_config;
 
 /* Diagnostics of one analysis() call (nuosc_ana.cpp:34-137).  The first nine

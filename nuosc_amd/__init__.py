@@ -28,6 +28,8 @@ NUOSC_OK, NUOSC_EINVAL, NUOSC_EHIP, NUOSC_ENOMEM, NUOSC_EUNSTABLE, NUOSC_ENODEV,
 PREC_FP64, PREC_FP32 = 64, 32
 VGRID_VORONOI, VGRID_EQUAL = 0, 1
 BC_PERIODIC, BC_OPEN_Z = 0, 1
+# advection schemes (NUOSC_ADV_*): the reference's default and its ADVEC_* variants
+ADV_CFD4, ADV_CFD2, ADV_UPWIND2, ADV_LOPSIDED, ADV_MUSCL, ADV_OFF = range(6)
 
 
 class NuOscError(RuntimeError):
@@ -44,6 +46,7 @@ class Config(C.Structure):
         ("cfl", C.c_double), ("ko", C.c_double), ("mu", C.c_double), ("pmo", C.c_double),
         ("renorm", C.c_int), ("vgrid", C.c_int), ("precision", C.c_int), ("bc", C.c_int),
         ("device", C.c_int), ("use_graph", C.c_int), ("zc", C.c_double),
+        ("ko_order", C.c_int), ("adv", C.c_int),
     ]
 
 
@@ -179,14 +182,15 @@ class NuOsc:
 
     def __init__(self, nv=4, gx=2, gz=2, x0=-1.0, x1=1.0, z0=-10.0, z1=10.0, dx=0.1, dz=0.1,
                  CFL=0.4, ko=0.0, mu=1.0, pmo=0.1, renorm=False, precision=PREC_FP64,
-                 vgrid_mode=VGRID_VORONOI, device=0, use_graph=True, bc=0, zc=0.0):
+                 vgrid_mode=VGRID_VORONOI, device=0, use_graph=True, bc=0, zc=0.0, ko_order=3, adv=ADV_CFD4):
         """bc: BC_PERIODIC (the reference's boundary) or BC_OPEN_Z (open z
         ends with the ipt-0 inflow pulse, x periodic); zc: centre of the ipt-0
-        profile (0 in the reference)."""
+        profile (0 in the reference); ko_order (3 or 5) and adv (ADV_*): the
+        discretisation (the reference's KO_ORD_3 / ADVEC_* build options)."""
         L = lib()
         cfg = Config(nn=nv, gx=gx, gz=gz, x0=x0, x1=x1, z0=z0, z1=z1, dx=dx, dz=dz, cfl=CFL, ko=ko,
                      mu=mu, pmo=pmo, renorm=int(renorm), vgrid=vgrid_mode, precision=precision, bc=bc,
-                     device=device, use_graph=int(use_graph), zc=zc)
+                     device=device, use_graph=int(use_graph), zc=zc, ko_order=ko_order, adv=adv)
         h = _vp()
         rc = L.nuosc_create(C.byref(cfg), C.byref(h))
         if rc != 0:

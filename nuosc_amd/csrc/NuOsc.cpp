@@ -27,7 +27,7 @@ void NuOsc::check(int rc, const char* what) {
 
 NuOsc::NuOsc(int px, int pz, int nv, int nphi, int gx, int gz, double x0, double x1, double z0, double z1,
              double dx_, double dz_, double CFL_, double ko_, int precision, int vgrid, bool use_graph, int device,
-             int bc, double zc)
+             int bc, double zc, int ko_order, int adv)
     : dx(dx_), dz(dz_), CFL(CFL_), ko(ko_) {
     (void)nphi;  // polar velocity grid only
     if (px != 1 || pz != 1)
@@ -51,6 +51,8 @@ NuOsc::NuOsc(int px, int pz, int nv, int nphi, int gx, int gz, double x0, double
     c.precision = precision;
     c.bc = bc;
     c.zc = zc;
+    c.ko_order = ko_order;
+    c.adv = adv;
     c.device = device;
     c.use_graph = use_graph ? 1 : 0;
     int rc = nuosc_create(&c, &sim_);
@@ -95,8 +97,12 @@ NuOsc::NuOsc(int px, int pz, int nv, int nphi, int gx, int gz, double x0, double
     else
         std::printf("   Boundary: Periodic\n");
     std::printf("   Velocity grid: Icosahedron sphere grid%s\n", vgrid == NUOSC_VGRID_EQUAL ? " (equal weights)" : "");
-    std::printf("   Dissipation: 3-th order KO w/ eps = %g\n", ko);
-    std::printf("   Advection: ON (Center-FD)\n");
+    // nuosc_class.cpp:492-503
+    if (ko_order == 5) std::printf("   Dissipation: 5-th order KO w/ eps = %g\n", ko);
+    else std::printf("   Dissipation: 3-th order KO w/ eps = %g\n", ko);
+    static const char* const kAdvName[] = {"ON (Center-FD)", "ON (Center-FD, 2nd order)", "ON (2nd-order upwind)",
+                                           "ON (4th-order lopsided)", "ON (MUSCL, superbee)", "OFF"};
+    std::printf("   Advection: %s\n", kAdvName[adv >= 0 && adv <= NUOSC_ADV_OFF ? adv : 0]);
     std::printf("   Vacuum term: pmo= %g theta= %g\n", pmo, theta);
 
     // nuosc_class.cpp:536-540

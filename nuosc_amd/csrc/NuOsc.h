@@ -11,7 +11,8 @@
 // quantity ("P3" or a field name) instead of holding a raw host pointer,
 // and read the device state at the time of the dump.
 // Added: precision, velocity-grid weights, boundary (periodic or open z
-// with an inflowing ipt-0 pulse centred at zc) and, beside analysis.dat
+// with an inflowing ipt-0 pulse centred at zc), the discretisation (the
+// reference's KO_ORD_3 / ADVEC_* build options: ko_order, adv) and, beside analysis.dat
 // (whose columns stay the reference's), growth.dat with |rho_ex| per
 // analysis call for linear growth rates.
 #pragma once
@@ -60,7 +61,8 @@ public:
 
     NuOsc(int px, int pz, int nv, int nphi, int gx, int gz, double x0, double x1, double z0, double z1, double dx,
           double dz, double CFL, double ko, int precision = NUOSC_PREC_FP64, int vgrid = NUOSC_VGRID_VORONOI,
-          bool use_graph = true, int device = 0, int bc = NUOSC_BC_PERIODIC, double zc = 0.0);
+          bool use_graph = true, int device = 0, int bc = NUOSC_BC_PERIODIC, double zc = 0.0, int ko_order = 3,
+          int adv = NUOSC_ADV_CFD4);
     ~NuOsc();
     NuOsc(const NuOsc&) = delete;
     NuOsc& operator=(const NuOsc&) = delete;

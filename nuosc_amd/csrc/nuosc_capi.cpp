@@ -317,7 +317,7 @@ int nuosc_abi_version(void) { return NUOSC_ABI_VERSION; }
 
 const char* nuosc_build_info(void) {
     return "libnuosc_hip: gfx950 RK4 stage kernels (fp64, fp32 + fp64 moments; periodic and open-z boundaries), "
-           "abi 2";
+           "abi 3";
 }
 
 int nuosc_vgrid(int nn, int mode, double* vx, double* vy, double* vz, double* vw) {
@@ -348,6 +348,11 @@ int nuosc_create(const nuosc_config* cfg, nuosc_sim** out) {
         return fail(nullptr, NUOSC_EINVAL, "bc must be NUOSC_BC_PERIODIC or NUOSC_BC_OPEN_Z");
     if (c.bc == NUOSC_BC_OPEN_Z && c.gz > 3)
         return fail(nullptr, NUOSC_EINVAL, "the open z boundary takes gz = 2 or 3");
+    if (c.ko_order != 0 && c.ko_order != 3 && c.ko_order != 5)
+        return fail(nullptr, NUOSC_EINVAL, "ko_order must be 3 or 5");
+    if (c.adv < NUOSC_ADV_CFD4 || c.adv > NUOSC_ADV_OFF) return fail(nullptr, NUOSC_EINVAL, "unknown advection scheme");
+    if (c.bc == NUOSC_BC_OPEN_Z && c.gz < scheme_radius(c.ko_order == 5, c.adv))
+        return fail(nullptr, NUOSC_EINVAL, "this discretisation has stencil radius 3: the open z boundary needs gz = 3");
     if (c.vgrid != NUOSC_VGRID_VORONOI && c.vgrid != NUOSC_VGRID_EQUAL)
         return fail(nullptr, NUOSC_EINVAL, "unknown velocity grid");
 
@@ -448,6 +453,10 @@ int nuosc_create(const nuosc_config* cfg, nuosc_sim** out) {
     s->geo.xrow = s->xrow;
     s->geo.twelve_dx = 12 * c.dx;
     s->geo.twelve_dz = 12 * c.dz;
+    s->geo.dx = c.dx;
+    s->geo.dz = c.dz;
+    s->geo.ko5 = c.ko_order == 5;
+    s->geo.adv = c.adv;
     s->geo.ko_x = -c.ko / c.dx / 16.0;  // nuosc_class.cpp:120-121
     s->geo.ko_z = -c.ko / c.dz / 16.0;
     s->geo.vx = s->d_vgrid;
@@ -1096,6 +1105,8 @@ int nuosc_batch_add(nuosc_batch* b, nuosc_sim* s) {
         return bfail(b, NUOSC_EINVAL, "member must use the batch's device and precision");
     if (s->cfg.renorm) return bfail(b, NUOSC_EINVAL, "renormalising runs cannot be batched");
     if (s->cfg.bc != NUOSC_BC_PERIODIC) return bfail(b, NUOSC_EINVAL, "only periodic runs can be batched");
+    if (s->geo.ko5 || s->geo.adv != NUOSC_ADV_CFD4)
+        return bfail(b, NUOSC_EINVAL, "only runs with the default discretisation can be batched");
     for (nuosc_sim* m : b->members)
         if (m == s) return bfail(b, NUOSC_EINVAL, "already a member");
     b->members.push_back(s);

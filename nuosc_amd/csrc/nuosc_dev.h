@@ -8,9 +8,9 @@
 //     copy[xs][zs][f][nvp],  xs = i + kXg (stored x row), zs = j + zoff
 //   so the 8 fields of a site are one contiguous block (srec = 8 nvp elements
 //   between site records) and a stored x row (nzs records) is contiguous.
-//   Each copy is its own allocation.  Stored x rows: kXg = 2 periodic
-//   ghost rows at each end (copies of interior rows nx-2, nx-1 and 0, 1,
-//   refreshed before every stage from the interior) and one spare row at
+//   Each copy is its own allocation.  Stored x rows: kXg = 3 periodic
+//   ghost rows at each end (copies of interior rows nx-3 .. nx-1 and
+//   0 .. 2, refreshed before every stage from the interior) and one spare row at
 //   xs = nx + 2 kXg, where stage-kernel lanes past the last x row store
 //   instead of branching; nxs = nx + 2 kXg + 1.  The x-neighbours of every
 //   wave are therefore one contiguous address range (a few stored rows, one
@@ -39,7 +39,11 @@ constexpr int kMomStride = 16;   // doubles per per-site moment total record (12
 constexpr int kMomRec = 12;      // doubles per (site, chunk) partial moment record
 constexpr int kNMom = 12;
 constexpr int kNAna = 12;        // analysis accumulators (see AnaIdx)
-constexpr int kXg = 2;           // stored periodic ghost x rows at each end
+constexpr int kXg = 3;           // stored periodic ghost x rows at each end (the widest stencil's radius)
+
+// advection schemes of the right-hand side (the reference's compile-time
+// variants, nuosc_class.cpp:127-183; kAdvCfd4 is its default)
+enum AdvScheme { kAdvCfd4 = 0, kAdvCfd2, kAdvUpwind2, kAdvLopsided, kAdvMuscl, kAdvOff };
 
 enum AnaIdx {
     ANA_SURV = 0, ANA_SURVB, ANA_AVGP, ANA_AVGPB, ANA_AM01, ANA_AM02, ANA_AM03,
@@ -57,6 +61,9 @@ struct Geometry {
     long long xrow;        // elements between stored x rows of one copy: nzs * srec
     double twelve_dx, twelve_dz;   // 12*dx, 12*dz (advection denominators)
     double ko_x, ko_z;     // -ko/dx/16, -ko/dz/16
+    double dx, dz;
+    int ko5;               // 5th-order Kreiss-Oliger (radius 3) instead of the 3rd-order default
+    int adv;               // AdvScheme
     const double *vx, *vy, *vz, *vw;   // device, length nv
     int bc;                // 0 periodic, 1 open in z
     int nzs, zoff;         // stored z rows per x row, stored row of interior z = 0
@@ -102,6 +109,10 @@ struct StagePlan {
     int zlen, nseg, nquads, ntask, nblocks;   // z-segment length and count, x-row quads, waves, blocks
 };
 
+// stencil radius of a discretisation: 3 for the 5th-order KO or the lopsided
+// advection, else 2
+inline int scheme_radius(int ko5, int adv) { return (ko5 || adv == kAdvLopsided) ? 3 : 2; }
+
 // x-rows per wavefront of the stage kernel (FP64 4 rows x 16 directions,
 // FP32 2 rows x 32 directions: one 128-byte row segment per field and row)
 int rows_per_wave(int esize);
@@ -123,9 +134,10 @@ hipError_t launch_stage(int stage, const Geometry& g, const Physics& ph, const S
                         const T* yin, const T* u, T* w, T* yout, const double* mtot, double* mpart,
                         double cw, double cy, hipStream_t s, double ca = 0.0, double cb = 0.0);
 
-// periodic x ghost rows of a state copy: stored rows 0, 1 <- interior rows
-// nx-2, nx-1 and rows nx+2, nx+3 <- interior 0, 1 (whole stored rows, z
-// ghost rows included).  Run on every stage input before the stage.
+// periodic x ghost rows of a state copy: stored rows 0 .. kXg-1 <- interior
+// rows nx-kXg .. nx-1 and rows nx+kXg .. nx+2kXg-1 <- interior 0 .. kXg-1
+// (whole stored rows, z ghost rows included).  Run on every stage input
+// before the stage.
 template <typename T>
 hipError_t launch_xghosts(const Geometry& g, T* y, hipStream_t s);
 

@@ -309,14 +309,53 @@ __device__ __forceinline__ double group_sum16(const double (&a)[16], const doubl
     }
 }
 
-// x rows staged per block of W waves: block rows -2 .. WR+1 (slots
-// 0 .. WR+3) and one slot that idle lanes write; D = L V directions each
-template <typename T, int R, int W>
+// x rows staged per block of W waves for a stencil of radius RAD: block rows
+// -RAD .. WR+RAD-1 (slots 0 .. WR+2RAD-1) and one slot that idle lanes
+// write; D = L V directions each
+template <typename T, int R, int W, int RAD>
 struct Slab {
     static constexpr int D = (kWave / R) * kVals<T>;
-    static constexpr int NS = W * R + 5;
+    static constexpr int NS = W * R + 2 * RAD + 1;
     T v[2][kNVar][NS * D];   // two z levels (by parity) of all 8 fields
 };
+
+// superbee limiter (nuosc_class.cpp:5-8)
+template <typename T>
+__device__ __forceinline__ T flimiter(T a, T b) {
+    const T sa = (T)((T)0 < a) - (T)(a < (T)0), sb = (T)((T)0 < b) - (T)(b < (T)0);
+    return (T)0.5 * (sa + sb) * fmax(fmin((T)2 * fabs(a), fabs(b)), fmin(fabs(a), (T)2 * fabs(b)));
+}
+
+// Advection term along one axis for the non-default schemes
+// (nuosc_class.cpp:127-178); m[k] / p[k] = the value k cells below / above,
+// f = the scheme's factor (set up per direction), s = sign of the velocity.
+template <typename T>
+__device__ __forceinline__ T adv_axis(int adv, T f, int s, const T (&m)[4], T c0, const T (&p)[4]) {
+    switch (adv) {
+        case kAdvCfd2:
+            return f * (p[1] - m[1]);
+        case kAdvUpwind2: {   // x[-2s] - 4 x[-s] + 3 x[0]
+            const T a2 = s > 0 ? m[2] : p[2], a1 = s > 0 ? m[1] : p[1];
+            return f * (a2 - (T)4 * a1 + (T)3 * c0);
+        }
+        case kAdvLopsided: {  // -x[-3s] + 6 x[-2s] - 18 x[-s] + 10 x[0] + 3 x[s]
+            const T a3 = s > 0 ? m[3] : p[3], a2 = s > 0 ? m[2] : p[2], a1 = s > 0 ? m[1] : p[1];
+            const T b1 = s > 0 ? p[1] : m[1];
+            return f * (-a3 + (T)6 * a2 - (T)18 * a1 + (T)10 * c0 + (T)3 * b1);
+        }
+        case kAdvMuscl: {     // f = -v/h
+            const T up = (T)(s > 0) * (c0 + (T)0.5 * flimiter(c0 - m[1], p[1] - c0) - m[1] -
+                                       (T)0.5 * flimiter(m[1] - m[2], c0 - m[1]));
+            const T dn = (T)(s < 0) * (p[1] - (T)0.5 * flimiter(p[1] - c0, p[2] - p[1]) - c0 +
+                                       (T)0.5 * flimiter(c0 - m[1], p[1] - c0));
+            return f * (up + dn);
+        }
+        case kAdvOff:
+            return (T)0;
+        default:              // 4th-order central
+            return f * ((m[2] - p[2]) - (T)8.0 * (m[1] - p[1]));
+    }
+}
 
 // Orders the LDS accesses of a block: every LDS write of every wave before it
 // is visible to every LDS read after it.  The fences cover LDS only, so
@@ -331,8 +370,9 @@ __device__ __forceinline__ void block_sync() {
 // chunk, z segment), quad group fastest; wave wib of the block takes quad
 // W * group + wib.  Lane (g, dl) of a wave handles x row i0 + g and the V
 // adjacent directions ch * D + dl * V + e, e < V.
-template <typename T, int R, int W, int STAGE, bool COMP>
-__device__ __forceinline__ void stage_body(const StageArgs<T>& a, int lblock, int wib, Slab<T, R, W>& slab) {
+template <typename T, int R, int W, int RAD, int STAGE, bool COMP>
+__device__ __forceinline__ void stage_body(const StageArgs<T>& a, int lblock, int wib, Slab<T, R, W, RAD>& slab) {
+    constexpr int NZW = 2 * RAD + 1;   // z window
     constexpr int V = kVals<T>;    // directions per lane (one 8-byte load)
     constexpr int L = kWave / R;   // lanes per row
     constexpr int D = L * V;       // directions per row chunk (one 128-byte line)
@@ -366,14 +406,15 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& a, int lblock, in
     const bool act = row_ok && p * V < nv;   // nv is a multiple of V
     const int pl = min(p, nv / V - 1);       // loads stay inside the directions
 
-    // The block's x rows ib-2 .. ib+4R+1 are the stored rows ib .. ib+4R+3
-    // (the periodic ghost rows make them contiguous); rows past the last
-    // interior row + 2 map to the spare stored row nx + 4.  One descriptor
-    // per state copy covers stored rows ib .. min(ib + 4R + 4, nxs - 1); a
-    // lane's stored row relative to ib goes into its vector offset.
+    // Block row b (x row ib + b) is stored row ib + b + kXg (the periodic
+    // ghost rows make the block's rows ib-RAD .. ib+WR+RAD-1 contiguous);
+    // rows past the last interior row + kXg map to the spare stored row
+    // nx + 2kXg.  One descriptor per state copy covers stored rows
+    // ib .. min(ib + WR + 2kXg, nxs - 1); a lane's stored row relative to ib
+    // goes into its vector offset.
     const int spare = nx + 2 * kXg - ib;     // spare stored row, relative to ib
     const unsigned long long woff = (unsigned long long)ib * xrowb;
-    const unsigned wbytes = (unsigned)(min(W * R + 5, a.g.nxs - ib)) * xrowb;
+    const unsigned wbytes = (unsigned)(min(W * R + 2 * kXg + 1, a.g.nxs - ib)) * xrowb;
     const Rows<T> yin{a.yin + woff, wbytes, rowb}, uu{a.u + woff, wbytes, rowb};
     const Rows<T> ww{a.w + woff, wbytes, rowb}, yout{a.yout + woff, wbytes, rowb};
     // stored z row of interior z index jj (periodic: wrapped; open: the ghost
@@ -389,15 +430,17 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& a, int lblock, in
     // stores the same bits to the same address as that direction's lane, so
     // the padding columns are never written.
     const unsigned vos = (unsigned)(row_ok ? own + kXg : spare) * xrowb + dof;
-    // Edge rows: wave k < 4 loads block row -2, -1, WR, WR+1 (k = 0..3) with
-    // its first L lanes.  The other lanes' offset lies past the descriptor's
-    // range, so they fetch nothing (the load returns zeros), and they write
-    // the idle slot.
-    const bool eln = g == 0 && wib < 4;
-    const int erow = wib < 2 ? wib : min(W * R + wib, spare);   // stored row relative to ib
+    // Edge rows: edge e = wib + 4g (row group g < 2 of wave wib, e < 2 RAD)
+    // loads block row e - RAD (e < RAD) or WR + e - RAD with its L lanes.  The
+    // other lanes' offset lies past the descriptor's range, so they fetch
+    // nothing (the load returns zeros), and they write the idle slot.
+    const int eidx = wib + 4 * g;
+    const bool eln = g < 2 && eidx < 2 * RAD;
+    const int eb = eidx < RAD ? eidx - RAD : W * R + eidx - RAD;   // block row
+    const int erow = min(eb + kXg, spare);                           // stored row relative to ib
     const unsigned vh = eln ? (unsigned)erow * xrowb + dof : wbytes;
-    const int slot_h = (eln ? (wib < 2 ? wib : W * R + wib) : W * R + 4) * D + dl * V;
-    const int slot_c = (own + 2) * D + dl * V;
+    const int slot_h = (eln ? eb + RAD : W * R + 2 * RAD) * D + dl * V;
+    const int slot_c = (own + RAD) * D + dl * V;
 
     // per-direction constants (nuosc_class.cpp:119-121, 180-181)
     double vxd[V], vyd[V], vzd[V], q[V][4];
@@ -409,14 +452,34 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& a, int lblock, in
         vyd[e] = a.g.vy[ve];
         vzd[e] = a.g.vz[ve];
         const double vwd = act ? a.g.vw[ve] : 0.0;
-        fx[e] = (T)(-vxd[e] / a.g.twelve_dx);
-        fz[e] = (T)(-vzd[e] / a.g.twelve_dz);
+        if constexpr (RAD == 2) {   // nuosc_class.cpp:180-181
+            fx[e] = (T)(-vxd[e] / a.g.twelve_dx);
+            fz[e] = (T)(-vzd[e] / a.g.twelve_dz);
+        } else {                    // the scheme's factor (nuosc_class.cpp:150-181)
+            const int sx = (0.0 < vxd[e]) - (vxd[e] < 0.0), sz = (0.0 < vzd[e]) - (vzd[e] < 0.0);
+            switch (a.g.adv) {
+                case kAdvCfd2: fx[e] = (T)(-vxd[e] / (2 * a.g.dx)); fz[e] = (T)(-vzd[e] / (2 * a.g.dz)); break;
+                case kAdvUpwind2:
+                    fx[e] = (T)(-sx * vxd[e] / (2 * a.g.dx));
+                    fz[e] = (T)(-sz * vzd[e] / (2 * a.g.dz));
+                    break;
+                case kAdvLopsided:
+                    fx[e] = (T)(-sx * vxd[e] / (12 * a.g.dx));
+                    fz[e] = (T)(-sz * vzd[e] / (12 * a.g.dz));
+                    break;
+                case kAdvMuscl: fx[e] = (T)(-vxd[e] / a.g.dx); fz[e] = (T)(-vzd[e] / a.g.dz); break;
+                default: fx[e] = (T)(-vxd[e] / a.g.twelve_dx); fz[e] = (T)(-vzd[e] / a.g.twelve_dz); break;
+            }
+        }
         q[e][0] = vwd;
         q[e][1] = vxd[e] * vwd;
         q[e][2] = vyd[e] * vwd;
         q[e][3] = vzd[e] * vwd;
     }
-    const T kx = (T)a.g.ko_x, kz = (T)a.g.ko_z;
+    // Kreiss-Oliger: order 3 -ko/(16h), order 5 -ko/(64h) (nuosc_class.cpp:114-121)
+    const T kx = (T)(RAD == 3 && a.g.ko5 ? 0.25 * a.g.ko_x : a.g.ko_x);
+    const T kz = (T)(RAD == 3 && a.g.ko5 ? 0.25 * a.g.ko_z : a.g.ko_z);
+    const int sxv = (0.0 < vxd[0]) - (vxd[0] < 0.0), szv = (0.0 < vzd[0]) - (vzd[0] < 0.0);
     const T mu2 = (T)a.mu2, mu = (T)a.mu, p2st = (T)a.p2st, p2ct = (T)a.p2ct;
     const T pmo = (T)a.pmo, ct2 = (T)a.ct2, st = (T)a.st;
     const T cw = (T)a.cw, cy = (T)a.cy, ca = (T)a.ca, cb = (T)a.cb;
@@ -431,7 +494,7 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& a, int lblock, in
     const int j0 = zs * a.zlen;
     const int j1 = min(nz, j0 + a.zlen);
 
-    T win[kNVar][5][V];   // own row, z = j-2 .. j+2
+    T win[kNVar][NZW][V]; // own row, z = j-RAD .. j+RAD
     T hreg[kNVar][V];     // edge row of z = j+2, staged into LDS in step j+1
     // velocity moments of the stage input at this lane's site (one value per
     // lane) for z = j, loaded at the end of step j-1
@@ -443,8 +506,8 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& a, int lblock, in
         return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rm, vom, sm, 0));
     };
 #pragma unroll
-    for (int qz = 0; qz < 4; ++qz) {
-        const unsigned so = zrow(j0 - 2 + qz) * siteb;
+    for (int qz = 0; qz < 2 * RAD; ++qz) {
+        const unsigned so = zrow(j0 - RAD + qz) * siteb;
 #pragma unroll
         for (int f = 0; f < kNVar; ++f) yin.ldv(yin.base, f, vo, so, win[f][qz]);
     }
@@ -471,7 +534,7 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& a, int lblock, in
 #pragma unroll
         for (int f = 0; f < kNVar; ++f)
 #pragma unroll
-            for (int e = 0; e < V; ++e) xs[lv][f][slot_c + e] = win[f][2][e];
+            for (int e = 0; e < V; ++e) xs[lv][f][slot_c + e] = win[f][RAD][e];
         // all rows of z = j are in level lv, and every wave is done with step
         // j-1, whose level receives the edge rows of z = j+1 (loaded in step
         // j-1; no wave reads that level before the next barrier)
@@ -480,13 +543,13 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& a, int lblock, in
         for (int f = 0; f < kNVar; ++f)
 #pragma unroll
             for (int e = 0; e < V; ++e) xs[lv ^ 1][f][slot_h + e] = hreg[f][e];
-        // ---- the step's loads, in the order they are consumed: own row z+2
-        // (stencil), u and w (update), edge row z+2 (staged in the next
+        // ---- the step's loads, in the order they are consumed: own row
+        // z+RAD (stencil), u and w (update), edge row z+2 (staged in the next
         // step); the moments of z+1 are loaded at the end of the step
         const unsigned long long by = yin.fresh_base();
         const unsigned s2 = zrow(j + 2) * siteb;
 #pragma unroll
-        for (int f = 0; f < kNVar; ++f) yin.ldv(by, f, vo, s2, win[f][4]);
+        for (int f = 0; f < kNVar; ++f) yin.ldv(by, f, vo, zrow(j + RAD) * siteb, win[f][2 * RAD]);
         // (scheduling barriers keep the groups in this issue order, so each
         // wait below covers only the groups needed so far)
         __builtin_amdgcn_sched_barrier(0);
@@ -515,8 +578,8 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& a, int lblock, in
             const T B = (T)row_project<4>(Ml, vxd[e], vyd[e], vzd[e]);
             const T C = (T)row_project<8>(Ml, vxd[e], vyd[e], vzd[e]);
 
-            const T ee = win[0][2][e], mm = win[1][2][e], emr = win[2][2][e], emi = win[3][2][e];
-            const T bee = win[4][2][e], bmm = win[5][2][e], bemr = win[6][2][e], bemi = win[7][2][e];
+            const T ee = win[0][RAD][e], mm = win[1][RAD][e], emr = win[2][RAD][e], emi = win[3][RAD][e];
+            const T bee = win[4][RAD][e], bmm = win[5][RAD][e], bemr = win[6][RAD][e], bemi = win[7][RAD][e];
             // interaction (nuosc_class.cpp:193-200)
             const T Iee = mu2 * (emi * A + emr * B);
             const T Iemr = -mu * ((ee - mm) * B + C * emi);
@@ -531,16 +594,35 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& a, int lblock, in
                                   pmo * (ct2 * bemr + st * (bee - bmm))};
 #pragma unroll
             for (int f = 0; f < kNVar; ++f) {
-                const T xm2 = xs[lv][f][slot_c - 2 * D + e], xm1 = xs[lv][f][slot_c - D + e];
-                const T xp1 = xs[lv][f][slot_c + D + e], xp2 = xs[lv][f][slot_c + 2 * D + e];
-                const T zm2 = win[f][0][e], zm1 = win[f][1][e], c0 = win[f][2][e], zp1 = win[f][3][e],
-                        zp2 = win[f][4][e];
-                // advection (nuosc_class.cpp:182-183)
-                const T adv = fz[e] * ((zm2 - zp2) - (T)8.0 * (zm1 - zp1)) +
-                              fx[e] * ((xm2 - xp2) - (T)8.0 * (xm1 - xp1));
-                // Kreiss-Oliger (nuosc_class.cpp:122-123)
-                const T ko = kz * (zm2 + zp2 - (T)4 * (zm1 + zp1) + (T)6 * c0) +
-                             kx * (xm2 + xp2 - (T)4 * (xm1 + xp1) + (T)6 * c0);
+                const T c0 = win[f][RAD][e];
+                T adv, ko;
+                if constexpr (RAD == 2) {   // the reference's default discretisation
+                    const T xm2 = xs[lv][f][slot_c - 2 * D + e], xm1 = xs[lv][f][slot_c - D + e];
+                    const T xp1 = xs[lv][f][slot_c + D + e], xp2 = xs[lv][f][slot_c + 2 * D + e];
+                    const T zm2 = win[f][0][e], zm1 = win[f][1][e], zp1 = win[f][3][e], zp2 = win[f][4][e];
+                    // advection (nuosc_class.cpp:182-183)
+                    adv = fz[e] * ((zm2 - zp2) - (T)8.0 * (zm1 - zp1)) + fx[e] * ((xm2 - xp2) - (T)8.0 * (xm1 - xp1));
+                    // Kreiss-Oliger (nuosc_class.cpp:122-123)
+                    ko = kz * (zm2 + zp2 - (T)4 * (zm1 + zp1) + (T)6 * c0) +
+                         kx * (xm2 + xp2 - (T)4 * (xm1 + xp1) + (T)6 * c0);
+                } else {                    // scheme chosen at run time (nuosc_class.cpp:112-183)
+                    T xm[4], xp[4], zm[4], zp[4];
+#pragma unroll
+                    for (int kk = 1; kk <= RAD; ++kk) {
+                        xm[kk] = xs[lv][f][slot_c - kk * D + e];
+                        xp[kk] = xs[lv][f][slot_c + kk * D + e];
+                        zm[kk] = win[f][RAD - kk][e];
+                        zp[kk] = win[f][RAD + kk][e];
+                    }
+                    xm[0] = xp[0] = zm[0] = zp[0] = c0;
+                    adv = adv_axis<T>(a.g.adv, fz[e], szv, zm, c0, zp) + adv_axis<T>(a.g.adv, fx[e], sxv, xm, c0, xp);
+                    if (a.g.ko5)
+                        ko = kz * (zm[3] + zp[3] - (T)6 * (zm[2] + zp[2]) + (T)15 * (zm[1] + zp[1]) - (T)20 * c0) +
+                             kx * (xm[3] + xp[3] - (T)6 * (xm[2] + xp[2]) + (T)15 * (xm[1] + xp[1]) - (T)20 * c0);
+                    else
+                        ko = kz * (zm[2] + zp[2] - (T)4 * (zm[1] + zp[1]) + (T)6 * c0) +
+                             kx * (xm[2] + xp[2] - (T)4 * (xm[1] + xp[1]) + (T)6 * c0);
+                }
                 const T k = loc[f] + adv + ko + vac[f];
                 if constexpr (STAGE == 0) {
                     yo[f][e] = k;
@@ -619,12 +701,9 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& a, int lblock, in
 #pragma unroll
         for (int f = 0; f < kNVar; ++f)
 #pragma unroll
-            for (int e = 0; e < V; ++e) {
-                win[f][0][e] = win[f][1][e];
-                win[f][1][e] = win[f][2][e];
-                win[f][2][e] = win[f][3][e];
-                win[f][3][e] = win[f][4][e];
-            }
+            for (int e = 0; e < V; ++e)
+#pragma unroll
+                for (int qz = 0; qz < NZW - 1; ++qz) win[f][qz][e] = win[f][qz + 1][e];
         // moments of z+1, issued last and into the variable the next step
         // reads (so no register copy waits for it); unconditional (a branch
         // here would make every later wait a full drain): the last row of the
@@ -637,17 +716,19 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& a, int lblock, in
 // Waves per SIMD the register allocation aims at: FP64 stages 0-5 fit 168
 // VGPRs (3 waves), the SSP-RK3 stages 6 and 7 would spill there (2 waves);
 // the FP32 stages fit 128 (4 waves).
-template <typename T, int STAGE>
+template <typename T, int STAGE, int RAD = 2>
 constexpr int stage_waves() {
-    return sizeof(T) == 4 ? 4 : (STAGE <= 5 ? 3 : 2);
+    return RAD == 3 ? (sizeof(T) == 4 ? 2 : 1) : (sizeof(T) == 4 ? 4 : (STAGE <= 5 ? 3 : 2));
 }
 
-template <typename T, int R, int STAGE, bool COMP>
-__global__ __launch_bounds__(64 * StageShape<T>::W) __attribute__((amdgpu_waves_per_eu(stage_waves<T, STAGE>()))) void k_stage(const StageArgs<T> a) {
+// RAD = 2: the reference's default discretisation; RAD = 3: any of the
+// schemes, chosen at run time (Geometry::ko5, Geometry::adv)
+template <typename T, int R, int STAGE, bool COMP, int RAD = 2>
+__global__ __launch_bounds__(64 * StageShape<T>::W) __attribute__((amdgpu_waves_per_eu(stage_waves<T, STAGE, RAD>()))) void k_stage(const StageArgs<T> a) {
     constexpr int W = StageShape<T>::W;
-    __shared__ Slab<T, R, W> slab;
+    __shared__ Slab<T, R, W, RAD> slab;
     const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    stage_body<T, R, W, STAGE, COMP>(a, xcd_remap(blockIdx.x, gridDim.x), wib, slab);
+    stage_body<T, R, W, RAD, STAGE, COMP>(a, xcd_remap(blockIdx.x, gridDim.x), wib, slab);
 }
 
 // Many independent runs in one launch: logical block -> (run, block of run)
@@ -656,13 +737,13 @@ template <typename T, int R, int STAGE, bool COMP>
 __global__ __launch_bounds__(64 * StageShape<T>::W) __attribute__((amdgpu_waves_per_eu(stage_waves<T, STAGE>()))) void k_stage_batch(
     const StageArgs<T>* __restrict__ runs, const int* __restrict__ bmap) {
     constexpr int W = StageShape<T>::W;
-    __shared__ Slab<T, R, W> slab;
+    __shared__ Slab<T, R, W, 2> slab;
     const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lb = xcd_remap(blockIdx.x, gridDim.x);
     const __attribute__((address_space(4))) int* cm = (const __attribute__((address_space(4))) int*)bmap;
     const int run = cm[2 * lb], blk = cm[2 * lb + 1];
     const StageArgs<T> a = runs[run];   // uniform, read-only: scalar loads
-    stage_body<T, R, W, STAGE, COMP>(a, blk, wib, slab);
+    stage_body<T, R, W, 2, STAGE, COMP>(a, blk, wib, slab);
 }
 
 int rows_per_wave(int esize) { return esize == 8 ? StageShape<double>::R : StageShape<float>::R; }
@@ -708,7 +789,7 @@ bool buffer_addressable(const Geometry& g, int esize) {
     const int R = rows_per_wave(esize), W = waves_per_block(esize);
     // the block's rows plus one: an idle lane's offset (the range's end) plus
     // a scalar offset must not wrap
-    const long long wave_rows = (std::min<long long>(W * R + 5, g.nxs) + 1) * g.xrow * esize;
+    const long long wave_rows = (std::min<long long>(W * R + 2 * kXg + 1, g.nxs) + 1) * g.xrow * esize;
     const long long part = (long long)(g.nx + 1) * g.nz * g.nch * kMomRec * (long long)sizeof(double);
     const long long tot = (long long)g.nx * g.nz * kMomStride * (long long)sizeof(double);
     return wave_rows < lim && part < lim && tot < lim;
@@ -754,24 +835,39 @@ hipError_t launch_stage(int stage, const Geometry& g, const Physics& ph, const S
     constexpr int R = StageShape<T>::R;
     if (g.lpr != dirs_per_chunk((int)sizeof(T)) || g.wpb != StageShape<T>::W) return hipErrorInvalidValue;
     dim3 grid(pl.nblocks), block(64 * StageShape<T>::W);
-    switch (stage) {
-        case 0: hipLaunchKernelGGL((k_stage<T, R, 0, COMP>), grid, block, 0, s, a); break;
-        case 1: hipLaunchKernelGGL((k_stage<T, R, 1, COMP>), grid, block, 0, s, a); break;
-        case 2: hipLaunchKernelGGL((k_stage<T, R, 2, COMP>), grid, block, 0, s, a); break;
-        case 3: hipLaunchKernelGGL((k_stage<T, R, 3, COMP>), grid, block, 0, s, a); break;
-        case 4: hipLaunchKernelGGL((k_stage<T, R, 4, COMP>), grid, block, 0, s, a); break;
-        case 5: hipLaunchKernelGGL((k_stage<T, R, 5, COMP>), grid, block, 0, s, a); break;
-        case 6: hipLaunchKernelGGL((k_stage<T, R, 6, COMP>), grid, block, 0, s, a); break;
-        case 7: hipLaunchKernelGGL((k_stage<T, R, 7, COMP>), grid, block, 0, s, a); break;
-        default: return hipErrorInvalidValue;
+    if (scheme_radius(g.ko5, g.adv) == 2 && g.adv == kAdvCfd4) {
+        switch (stage) {
+            case 0: hipLaunchKernelGGL((k_stage<T, R, 0, COMP>), grid, block, 0, s, a); break;
+            case 1: hipLaunchKernelGGL((k_stage<T, R, 1, COMP>), grid, block, 0, s, a); break;
+            case 2: hipLaunchKernelGGL((k_stage<T, R, 2, COMP>), grid, block, 0, s, a); break;
+            case 3: hipLaunchKernelGGL((k_stage<T, R, 3, COMP>), grid, block, 0, s, a); break;
+            case 4: hipLaunchKernelGGL((k_stage<T, R, 4, COMP>), grid, block, 0, s, a); break;
+            case 5: hipLaunchKernelGGL((k_stage<T, R, 5, COMP>), grid, block, 0, s, a); break;
+            case 6: hipLaunchKernelGGL((k_stage<T, R, 6, COMP>), grid, block, 0, s, a); break;
+            case 7: hipLaunchKernelGGL((k_stage<T, R, 7, COMP>), grid, block, 0, s, a); break;
+            default: return hipErrorInvalidValue;
+        }
+    } else {
+        switch (stage) {
+            case 0: hipLaunchKernelGGL((k_stage<T, R, 0, COMP, 3>), grid, block, 0, s, a); break;
+            case 1: hipLaunchKernelGGL((k_stage<T, R, 1, COMP, 3>), grid, block, 0, s, a); break;
+            case 2: hipLaunchKernelGGL((k_stage<T, R, 2, COMP, 3>), grid, block, 0, s, a); break;
+            case 3: hipLaunchKernelGGL((k_stage<T, R, 3, COMP, 3>), grid, block, 0, s, a); break;
+            case 4: hipLaunchKernelGGL((k_stage<T, R, 4, COMP, 3>), grid, block, 0, s, a); break;
+            case 5: hipLaunchKernelGGL((k_stage<T, R, 5, COMP, 3>), grid, block, 0, s, a); break;
+            case 6: hipLaunchKernelGGL((k_stage<T, R, 6, COMP, 3>), grid, block, 0, s, a); break;
+            case 7: hipLaunchKernelGGL((k_stage<T, R, 7, COMP, 3>), grid, block, 0, s, a); break;
+            default: return hipErrorInvalidValue;
+        }
     }
     return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
-// periodic x ghost rows of a stage input: stored rows 0, 1, nx+2, nx+3 are
-// copies of the stored rows of interior x rows nx-2, nx-1, 0, 1 (whole stored
-// rows, z ghost rows included), moved as 16-byte words
+// periodic x ghost rows of a stage input: stored rows 0 .. kXg-1 and
+// nx+kXg .. nx+2kXg-1 are copies of the stored rows of interior x rows
+// nx-kXg .. nx-1 and 0 .. kXg-1 (whole stored rows, z ghost rows included),
+// moved as 16-byte words
 // ---------------------------------------------------------------------------
 struct XgArgs {
     unsigned long long y;   // state copy
@@ -792,13 +888,13 @@ XgArgs xg_args(const Geometry& g, const void* y, int esize) {
     a.nx = g.nx;
     return a;
 }
-long long xg_blocks(long long words) { return 4 * ((words + 256 * kXgPerThread - 1) / (256 * kXgPerThread)); }
+long long xg_blocks(long long words) { return 2 * kXg * ((words + 256 * kXgPerThread - 1) / (256 * kXgPerThread)); }
 
 __device__ __forceinline__ void xg_body(const XgArgs& a, long long blk) {
     const long long per = (a.words + 256 * kXgPerThread - 1) / (256 * kXgPerThread);
-    const int k = (int)(blk / per);                 // ghost row 0..3
+    const int k = (int)(blk / per);                 // ghost row 0 .. 2kXg-1
     const long long c = blk % per;
-    const int dst = k < 2 ? k : a.nx + k;           // stored rows 0, 1, nx+2, nx+3
+    const int dst = k < kXg ? k : a.nx + k;         // stored rows 0 .. kXg-1, nx+kXg .. nx+2kXg-1
     const int src = wrapi(dst - kXg, a.nx) + kXg;   // stored row of its interior x row
     uint4* d = reinterpret_cast<uint4*>(a.y) + dst * a.rowstride;
     const uint4* sp = reinterpret_cast<const uint4*>(a.y) + src * a.rowstride;

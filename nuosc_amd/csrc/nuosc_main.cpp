@@ -45,6 +45,8 @@ int main(int argc, char* argv[]) {
     int precision = NUOSC_PREC_FP64, vgrid = NUOSC_VGRID_VORONOI, snap_p3 = 0, use_graph = 1, device = 0;
     int bc = NUOSC_BC_PERIODIC, srk3 = 0;
     double zc = 0.0;
+    int ko_order = 3, adv = NUOSC_ADV_CFD4;
+    bool gz_set = false;
     unsigned long long seed = 12345;
 
     for (int t = 1; t < argc; t++) {
@@ -60,7 +62,7 @@ int main(int argc, char* argv[]) {
         else if (!std::strcmp(a, "--dx")) dx = std::atof(val());
         else if (!std::strcmp(a, "--zmax")) { z1 = std::atof(val()); z0 = -z1; x1 = z1; x0 = z0; }
         else if (!std::strcmp(a, "--xmax")) { x1 = std::atof(val()); x0 = -x1; }
-        else if (!std::strcmp(a, "--gz")) { gz = (int)std::atof(val()); gx = gz; }
+        else if (!std::strcmp(a, "--gz")) { gz = (int)std::atof(val()); gx = gz; gz_set = true; }
         else if (!std::strcmp(a, "--cfl")) cfl = std::atof(val());
         else if (!std::strcmp(a, "--nv")) nv_in = std::atoi(val());
         else if (!std::strcmp(a, "--nphi")) nphi = std::atoi(val());
@@ -104,6 +106,15 @@ int main(int argc, char* argv[]) {
         }
         else if (!std::strcmp(a, "--zc")) zc = std::atof(val());
         else if (!std::strcmp(a, "--srk3")) srk3 = std::atoi(val());
+        else if (!std::strcmp(a, "--ko_order")) ko_order = std::atoi(val());
+        else if (!std::strcmp(a, "--adv")) {
+            const char* m = val();
+            static const char* const names[] = {"cfd4", "cfd2", "upwind2", "lopsided", "muscl", "off"};
+            int k = 0;
+            while (k <= NUOSC_ADV_OFF && std::strcmp(m, names[k])) ++k;
+            if (k > NUOSC_ADV_OFF) { std::printf("Unreconganized parameters %s!\n", m); std::exit(0); }
+            adv = k;
+        }
         else {
             // nuosc.cpp:135-137 (message and exit status kept)
             std::printf("Unreconganized parameters %s!\n", a);
@@ -111,9 +122,11 @@ int main(int argc, char* argv[]) {
         }
     }
 
+    // nuosc.cpp:57-61: ghost width 3 for the radius-3 stencils unless given
+    if (!gz_set && (ko_order == 5 || adv == NUOSC_ADV_LOPSIDED)) gx = gz = 3;
     try {
         NuOsc state(px, pz, nv_in, nphi, gx, gz, x0, x1, z0, z1, dx, dz, cfl, ko, precision, vgrid, use_graph != 0,
-                    device, bc, zc);
+                    device, bc, zc, ko_order, adv);
         const long long lpts = state.grid.get_lpts();
         state.set_mu(mu);
         state.set_pmo(pmo);

@@ -200,6 +200,8 @@ orc_sim* orc_create(int nn, int gx, int gz, double x0, double x1, double z0, dou
     s->cfl = cfl;
     s->dt = dz * cfl;
     s->ko = ko;
+    s->ko_order = 3;
+    s->adv = ORC_ADV_CFD4;
     s->phy_time = 0.0;
     /* nuosc_class.h:120-126 */
     const double theta = 37 * M_PI / 180.;
@@ -435,7 +437,72 @@ void orc_fill_ghosts(orc_sim* s, double* const f[ORC_NVAR], double t) {
 }
 
 /* ------------------------------------------------------------------------- */
-/* Right-hand side (nuosc_class.cpp:11-307, default compile-time branches)    */
+/* Discretisation variants (nuosc_class.cpp:5-8, 112-183, nuosc_class.h:95)  */
+/* ------------------------------------------------------------------------- */
+int orc_scheme_radius(int ko_order, int adv) {
+    return (ko_order == 5 || adv == ORC_ADV_LOPSIDED) ? 3 : 2;
+}
+
+int orc_set_scheme(orc_sim* s, int ko_order, int adv) {
+    if (ko_order != 3 && ko_order != 5) return -1;
+    if (adv < ORC_ADV_CFD4 || adv > ORC_ADV_OFF) return -1;
+    const int r = orc_scheme_radius(ko_order, adv);
+    if (s->gx < r || s->gz < r) return -1;
+    s->ko_order = ko_order;
+    s->adv = adv;
+    return 0;
+}
+
+static int sgn_d(double a) { return (0.0 < a) - (a < 0.0); }
+
+/* superbee limiter (nuosc_class.cpp:5-8) */
+static double flimiter(double a, double b) {
+    return 0.5 * (sgn_d(a) + sgn_d(b)) * fmax(fmin(2 * fabs(a), fabs(b)), fmin(fabs(a), 2 * fabs(b)));
+}
+
+/* Advection term of one field at one point along one axis, x = pointer to the
+ * point, st = stride along the axis, v = velocity component, h = spacing
+ * (nuosc_class.cpp:127-183, non-default branches). */
+static double adv_axis(int adv, const double* x, long st, double v, double h) {
+    switch (adv) {
+        case ORC_ADV_CFD2: {
+            const double f = -v / (2 * h);
+            return f * (x[st] - x[-st]);
+        }
+        case ORC_ADV_UPWIND2: {
+            const int sg = sgn_d(v);
+            const double f = -sg * v / (2 * h);
+            return f * (x[-2 * sg * st] - 4 * x[-sg * st] + 3 * x[0]);
+        }
+        case ORC_ADV_LOPSIDED: {
+            const int sg = sgn_d(v);
+            const double f = -sg * v / (12 * h);
+            return f * (-x[-3 * sg * st] + 6 * x[-2 * sg * st] - 18 * x[-sg * st] + 10 * x[0] + 3 * x[sg * st]);
+        }
+        case ORC_ADV_MUSCL: {
+            const double up = (v > 0) * (x[0] + 0.5 * flimiter(x[0] - x[-st], x[st] - x[0]) - x[-st] -
+                                         0.5 * flimiter(x[-st] - x[-2 * st], x[0] - x[-st]));
+            const double dn = (v < 0) * (x[st] - 0.5 * flimiter(x[st] - x[0], x[2 * st] - x[st]) - x[0] +
+                                         0.5 * flimiter(x[0] - x[-st], x[st] - x[0]));
+            return -v / h * (up + dn);
+        }
+        case ORC_ADV_OFF:
+            return 0.0;
+        default: {  /* 4th-order central */
+            const double f = -v / (12 * h);
+            return f * ((x[-2 * st] - x[2 * st]) - 8.0 * (x[-st] - x[st]));
+        }
+    }
+}
+
+/* Kreiss-Oliger term of order 5 along one axis, eps = -ko/h/64
+ * (nuosc_class.cpp:112-117) */
+static double ko5_axis(const double* x, long st, double eps) {
+    return eps * (x[-3 * st] + x[3 * st] - 6 * (x[-2 * st] + x[2 * st]) + 15 * (x[-st] + x[st]) - 20 * x[0]);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Right-hand side (nuosc_class.cpp:11-307)                                   */
 /* ------------------------------------------------------------------------- */
 void orc_calrhs(const orc_sim* s, double* const out[ORC_NVAR], double* const in[ORC_NVAR]) {
     const int nx = s->nx, nz = s->nz, nv = s->nv;
@@ -444,9 +511,12 @@ void orc_calrhs(const orc_sim* s, double* const out[ORC_NVAR], double* const in[
     const double* vx = s->vx; const double* vy = s->vy;
     const double* vz = s->vz; const double* vw = s->vw;
     const double mu = s->mu, pmo = s->pmo, ct = s->ct, st = s->st;
-    /* nuosc_class.cpp:120-121 */
+    /* nuosc_class.cpp:120-121 (order 3), 114-115 (order 5) */
     const double ko_eps_x = -s->ko / s->dx / 16.0;
     const double ko_eps_z = -s->ko / s->dz / 16.0;
+    const double ko5_eps_x = -s->ko / s->dx / 64.0;
+    const double ko5_eps_z = -s->ko / s->dz / 64.0;
+    const int dflt = s->ko_order != 5 && s->adv == ORC_ADV_CFD4;
 
 #pragma omp parallel for collapse(2) schedule(static)
     for (int i = 0; i < nx; ++i)
@@ -467,10 +537,13 @@ void orc_calrhs(const orc_sim* s, double* const out[ORC_NVAR], double* const in[
                 /* nuosc_class.cpp:180-183 */
                 const double factor_x = -vx[v] / (12 * s->dx);
                 const double factor_z = -vz[v] / (12 * s->dz);
-#define ADV_FD(x) (factor_z * (((x)[-2 * snv] - (x)[2 * snv]) - 8.0 * ((x)[-snv] - (x)[snv])) + \
-                   factor_x * (((x)[-2 * nzv] - (x)[2 * nzv]) - 8.0 * ((x)[-nzv] - (x)[nzv])))
-#define KO_FD(x) (ko_eps_z * ((x)[-2 * snv] + (x)[2 * snv] - 4 * ((x)[-snv] + (x)[snv]) + 6 * (x)[0]) + \
-                  ko_eps_x * ((x)[-2 * nzv] + (x)[2 * nzv] - 4 * ((x)[-nzv] + (x)[nzv]) + 6 * (x)[0]))
+#define ADV_FD(x) (dflt ? (factor_z * (((x)[-2 * snv] - (x)[2 * snv]) - 8.0 * ((x)[-snv] - (x)[snv])) + \
+                          factor_x * (((x)[-2 * nzv] - (x)[2 * nzv]) - 8.0 * ((x)[-nzv] - (x)[nzv]))) \
+                       : (adv_axis(s->adv, (x), snv, vz[v], s->dz) + adv_axis(s->adv, (x), nzv, vx[v], s->dx)))
+#define KO_FD(x) (s->ko_order != 5 \
+                      ? (ko_eps_z * ((x)[-2 * snv] + (x)[2 * snv] - 4 * ((x)[-snv] + (x)[snv]) + 6 * (x)[0]) + \
+                         ko_eps_x * ((x)[-2 * nzv] + (x)[2 * nzv] - 4 * ((x)[-nzv] + (x)[nzv]) + 6 * (x)[0])) \
+                      : (ko5_axis((x), snv, ko5_eps_z) + ko5_axis((x), nzv, ko5_eps_x)))
 #define INP(a) ((a)[0] - (a)[1] * vx[v] - (a)[2] * vy[v] - (a)[3] * vz[v])
                 const double* ee = &in[0][ijv];
                 const double* mm = &in[1][ijv];

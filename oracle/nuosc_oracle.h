@@ -83,7 +83,19 @@ typedef struct orc_sim {
     double zc;                 /* centre of the ipt-0 profile (the reference uses 0) */
     int inflow_on;             /* inflow pulse active (fill_init with ipt 0) */
     double inflow_eps0, inflow_sigma;
+    /* discretisation of calRHS (the reference's compile-time variants,
+     * nuosc_class.cpp:112-183): Kreiss-Oliger order 3 (KO_ORD_3, default) or
+     * 5, and the advection scheme ORC_ADV_* */
+    int ko_order;
+    int adv;
 } orc_sim;
+
+#define ORC_ADV_CFD4 0      /* 4th-order central (default) */
+#define ORC_ADV_CFD2 1      /* ADVEC_CFD2O: 2nd-order central */
+#define ORC_ADV_UPWIND2 2   /* ADVEC_UPWIND2O: 2nd-order upwind */
+#define ORC_ADV_LOPSIDED 3  /* ADVEC_LOPESIDE: 4th-order lopsided */
+#define ORC_ADV_MUSCL 4     /* ADVEC_MUSCL: flux-limited (superbee) */
+#define ORC_ADV_OFF 5       /* ADVEC_OFF */
 
 #define ORC_BC_PERIODIC 0
 #define ORC_BC_OPEN_Z 1
@@ -100,6 +112,12 @@ void orc_set_physics(orc_sim* s, double mu, double pmo, int renorm);
 /* Boundary mode and ipt-0 profile centre; call before orc_fill_init.  The open
  * mode needs nz >= 3.  Returns 0, or -1 on bad input. */
 int orc_set_boundary(orc_sim* s, int bc, double zc);
+/* Discretisation (ko_order 3 or 5, adv ORC_ADV_*); the stencil radius (3 for
+ * KO 5 and the lopsided scheme, else 2) must not exceed gx and gz.  Returns 0,
+ * or -1 on bad input. */
+int orc_set_scheme(orc_sim* s, int ko_order, int adv);
+/* stencil radius of a discretisation */
+int orc_scheme_radius(int ko_order, int adv);
 
 /* nuosc_init.cpp:17-137.  ipt 0,2,3 as the reference; ipt 5 = seeded complex
  * perturbation (counter-based RNG keyed by seed and the interior index).

@@ -47,6 +47,7 @@ class _Sim(C.Structure):
         ("analysis_lines", C.c_int),
         ("bc", C.c_int), ("zc", C.c_double), ("inflow_on", C.c_int),
         ("inflow_eps0", C.c_double), ("inflow_sigma", C.c_double),
+        ("ko_order", C.c_int), ("adv", C.c_int),
     ]
 
 
@@ -69,6 +70,8 @@ def lib():
         L.orc_periodic_ghosts.argtypes = [C.POINTER(_Sim), dp * 8]
         L.orc_set_boundary.argtypes = [C.POINTER(_Sim), C.c_int, C.c_double]
         L.orc_set_boundary.restype = C.c_int
+        L.orc_set_scheme.argtypes = [C.POINTER(_Sim), C.c_int, C.c_int]
+        L.orc_set_scheme.restype = C.c_int
         L.orc_open_ghosts.argtypes = [C.POINTER(_Sim), dp * 8, C.c_double]
         L.orc_fill_ghosts.argtypes = [C.POINTER(_Sim), dp * 8, C.c_double]
         L.orc_calrhs.argtypes = [C.POINTER(_Sim), dp * 8, dp * 8]
@@ -123,9 +126,11 @@ class OracleSim:
 
     BUF_STAT, BUF_RHS, BUF_PRE, BUF_COR = 0, 1, 2, 3
     BC_PERIODIC, BC_OPEN_Z = 0, 1
+    # advection schemes (ORC_ADV_*)
+    ADV_CFD4, ADV_CFD2, ADV_UPWIND2, ADV_LOPSIDED, ADV_MUSCL, ADV_OFF = range(6)
 
     def __init__(self, nn=4, gx=2, gz=2, x0=-3.2, x1=3.2, z0=-3.2, z1=3.2, dx=0.1, dz=0.1,
-                 cfl=0.4, ko=0.1, mu=1.0, pmo=0.0, renorm=0, vgrid_mode=0, bc=0, zc=0.0):
+                 cfl=0.4, ko=0.1, mu=1.0, pmo=0.0, renorm=0, vgrid_mode=0, bc=0, zc=0.0, ko_order=3, adv=0):
         L = lib()
         p = L.orc_create(nn, gx, gz, x0, x1, z0, z1, dx, dz, cfl, ko, vgrid_mode)
         if not p:
@@ -135,6 +140,9 @@ class OracleSim:
         if L.orc_set_boundary(p, bc, zc) != 0:
             self.close()
             raise ValueError("orc_set_boundary rejected the configuration")
+        if L.orc_set_scheme(p, ko_order, adv) != 0:
+            self.close()
+            raise ValueError("orc_set_scheme rejected the configuration (ghost width below the stencil radius?)")
 
     def close(self):
         if self._p:
