@@ -212,7 +212,8 @@ orc_sim* orc_create(int nn, int gx, int gz, double x0, double x1, double z0, dou
     s->renorm = 0;
     s->G0 = (double*)calloc(s->nv, sizeof(double));
     s->G0b = (double*)calloc(s->nv, sizeof(double));
-    for (int f = 0; f < ORC_NVAR; ++f) {
+    s->nvar = ORC_NVAR2;
+    for (int f = 0; f < ORC_NVAR2; ++f) {
         s->stat[f] = (double*)calloc(s->lpts, sizeof(double));
         s->rhs[f] = (double*)calloc(s->lpts, sizeof(double));
         s->pre[f] = (double*)calloc(s->lpts, sizeof(double));
@@ -335,6 +336,12 @@ int orc_fill_init(orc_sim* s, int ipt, double alpha, double eps0, double sigma, 
                     s->stat[7][ijv] = 0.0;
                 }
     }
+    /* the reference's fillInitValue writes the 8 two-flavour fields only; the
+     * tau fields keep their construction value, zero (nuosc_class.h:24-46) */
+    for (int q = ORC_NVAR2; q < s->nvar; ++q)
+        for (int i = 0; i < s->nx; ++i)
+            for (int j = 0; j < s->nz; ++j)
+                memset(s->stat[q] + orc_idx(s, i, j, 0), 0, sizeof(double) * nv);
     s->n_nue0[0] = n00 * s->ds_L;
     s->n_nue0[1] = n01 * s->ds_L;
     s->phy_time = 0.0;
@@ -358,8 +365,8 @@ void orc_periodic_ghosts(orc_sim* s, double* const f[ORC_NVAR]) {
             for (int v = 0; v < nv; ++v) {
                 long l0 = orc_idx(s, i, -j - 1, v), l1 = orc_idx(s, i, nz - j - 1, v);
                 long r0 = orc_idx(s, i, nz + j, v), r1 = orc_idx(s, i, j, v);
-                for (int q = 0; q < ORC_NVAR; ++q) f[q][l0] = f[q][l1];
-                for (int q = 0; q < ORC_NVAR; ++q) f[q][r0] = f[q][r1];
+                for (int q = 0; q < s->nvar; ++q) f[q][l0] = f[q][l1];
+                for (int q = 0; q < s->nvar; ++q) f[q][r0] = f[q][r1];
             }
     for (int i = 0; i < gx; ++i)
 #pragma omp parallel for
@@ -367,8 +374,8 @@ void orc_periodic_ghosts(orc_sim* s, double* const f[ORC_NVAR]) {
             for (int v = 0; v < nv; ++v) {
                 long l0 = orc_idx(s, -i - 1, j, v), l1 = orc_idx(s, nx - i - 1, j, v);
                 long r0 = orc_idx(s, nx + i, j, v), r1 = orc_idx(s, i, j, v);
-                for (int q = 0; q < ORC_NVAR; ++q) f[q][l0] = f[q][l1];
-                for (int q = 0; q < ORC_NVAR; ++q) f[q][r0] = f[q][r1];
+                for (int q = 0; q < s->nvar; ++q) f[q][l0] = f[q][l1];
+                for (int q = 0; q < s->nvar; ++q) f[q][r0] = f[q][r1];
             }
 }
 
@@ -408,11 +415,12 @@ void orc_open_ghosts(orc_sim* s, double* const f[ORC_NVAR], double t) {
                         f[5][g] = 0.5 * G0b * (1.0 - p3o);
                         f[6][g] = 0.5 * G0b * a;
                         f[7][g] = 0.0;
+                        for (int k = ORC_NVAR2; k < s->nvar; ++k) f[k][g] = 0.0;
                     } else {
                         const long c0 = orc_idx(s, i, e0, v), c1 = orc_idx(s, i, e0 + dir, v),
                                    c2 = orc_idx(s, i, e0 + 2 * dir, v);
                         const double* w = W[q - 1];
-                        for (int k = 0; k < ORC_NVAR; ++k)
+                        for (int k = 0; k < s->nvar; ++k)
                             f[k][g] = w[0] * f[k][c0] + w[1] * f[k][c1] + w[2] * f[k][c2];
                     }
                 }
@@ -424,8 +432,8 @@ void orc_open_ghosts(orc_sim* s, double* const f[ORC_NVAR], double t) {
             for (int v = 0; v < nv; ++v) {
                 long l0 = orc_idx(s, -i - 1, j, v), l1 = orc_idx(s, nx - i - 1, j, v);
                 long r0 = orc_idx(s, nx + i, j, v), r1 = orc_idx(s, i, j, v);
-                for (int k = 0; k < ORC_NVAR; ++k) f[k][l0] = f[k][l1];
-                for (int k = 0; k < ORC_NVAR; ++k) f[k][r0] = f[k][r1];
+                for (int k = 0; k < s->nvar; ++k) f[k][l0] = f[k][l1];
+                for (int k = 0; k < s->nvar; ++k) f[k][r0] = f[k][r1];
             }
 }
 
@@ -450,6 +458,61 @@ int orc_set_scheme(orc_sim* s, int ko_order, int adv) {
     if (s->gx < r || s->gz < r) return -1;
     s->ko_order = ko_order;
     s->adv = adv;
+    return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Three flavours (N_FLAVOR_3, common.h:10-16)                                */
+/* ------------------------------------------------------------------------- */
+/* nuosc_class.cpp:543-569, as written there: mass splittings times 1267*2,
+ * mixing angles given as degrees/180 and used as radians (cos(33.82/180)),
+ * delta_cp = 0.  The names follow the reference: "hmtr" is the (e,tau)
+ * element of U diag(0, dm21, dm31) U^T and "hter" the (mu,tau) element. */
+void orc_vacuum_h3(double hv[9]) {
+    const double d21 = 7.39e-5 * 1267. * 2., d31 = 2.5229e-3 * 1267. * 2.;
+    const double a12 = 33.82 / 180., a13 = 8.61 / 180., a23 = 48.3 / 180., dcp = 0.;
+    const double c12 = cos(a12), s12 = sin(a12), c13 = cos(a13), s13 = sin(a13), c23 = cos(a23), s23 = sin(a23);
+    const double ue2 = s12 * c13;
+    const double um2r = c12 * c23 - s12 * s13 * s23 * cos(dcp);
+    const double ut2r = -c12 * s23 - s12 * s13 * c23 * cos(dcp);
+    const double um2i = -s12 * s13 * s23 * sin(dcp);
+    const double ut2i = -s12 * s13 * c23 * sin(dcp);
+    const double ue3r = s13 * cos(dcp);
+    const double ue3i = -s13 * sin(dcp);
+    const double um3 = c13 * s23;
+    const double ut3 = c13 * c23;
+    double ee = d21 * (ue2 * ue2) + d31 * (s13 * s13);
+    double mm = d21 * (um2r * um2r + um2i * um2i) + d31 * (um3 * um3);
+    double tt = d21 * (ut2r * ut2r + ut2i * ut2i) + d31 * (ut3 * ut3);
+    hv[3] = d21 * (ue2 * um2r) + d31 * (ue3r * um3);
+    hv[4] = d21 * (-ue2 * um2i) + d31 * (ue3i * um3);
+    hv[5] = d21 * (ue2 * ut2r) + d31 * (ue3r * ut3);
+    hv[6] = d21 * (-ue2 * ut2i) + d31 * (ue3i * ut3);
+    hv[7] = d21 * (um2r * ut2r + um2i * ut2i) + d31 * (um3 * ut3);
+    hv[8] = d21 * (ut2r * um2i - um2r * ut2i);
+    const double tr = (ee + mm + tt) / 3.0;
+    hv[0] = ee - tr;
+    hv[1] = mm - tr;
+    hv[2] = tt - tr;
+}
+
+int orc_set_flavors(orc_sim* s, int nflavor) {
+    if (nflavor == 2) {
+        s->nvar = ORC_NVAR2;
+        return 0;
+    }
+    if (nflavor != 3) return -1;
+    for (int f = ORC_NVAR2; f < ORC_NVAR3; ++f) {
+        if (!s->stat[f]) {
+            s->stat[f] = (double*)calloc(s->lpts, sizeof(double));
+            s->rhs[f] = (double*)calloc(s->lpts, sizeof(double));
+            s->pre[f] = (double*)calloc(s->lpts, sizeof(double));
+            s->cor[f] = (double*)calloc(s->lpts, sizeof(double));
+        }
+        if (!s->stat[f] || !s->rhs[f] || !s->pre[f] || !s->cor[f]) return -1;
+    }
+    s->nvar = ORC_NVAR3;
+    orc_vacuum_h3(s->hv);
     return 0;
 }
 
@@ -504,7 +567,120 @@ static double ko5_axis(const double* x, long st, double eps) {
 /* ------------------------------------------------------------------------- */
 /* Right-hand side (nuosc_class.cpp:11-307)                                   */
 /* ------------------------------------------------------------------------- */
+/* Stencil terms of the point x (a pointer into a ghosted field), with the
+ * locals of the functions below: advection (nuosc_class.cpp:127-183) and
+ * Kreiss-Oliger (:112-123) along z (stride snv) and x (stride nzv), and the
+ * projection of a moment vector onto direction v (INP, :187). */
+#define ADV_FD(x) (dflt ? (factor_z * (((x)[-2 * snv] - (x)[2 * snv]) - 8.0 * ((x)[-snv] - (x)[snv])) + \
+                          factor_x * (((x)[-2 * nzv] - (x)[2 * nzv]) - 8.0 * ((x)[-nzv] - (x)[nzv]))) \
+                       : (adv_axis(s->adv, (x), snv, vz[v], s->dz) + adv_axis(s->adv, (x), nzv, vx[v], s->dx)))
+#define KO_FD(x) (s->ko_order != 5 \
+                      ? (ko_eps_z * ((x)[-2 * snv] + (x)[2 * snv] - 4 * ((x)[-snv] + (x)[snv]) + 6 * (x)[0]) + \
+                         ko_eps_x * ((x)[-2 * nzv] + (x)[2 * nzv] - 4 * ((x)[-nzv] + (x)[nzv]) + 6 * (x)[0])) \
+                      : (ko5_axis((x), snv, ko5_eps_z) + ko5_axis((x), nzv, ko5_eps_x)))
+#define INP(a) ((a)[0] - (a)[1] * vx[v] - (a)[2] * vy[v] - (a)[3] * vz[v])
+
+/* Three flavours: moments nuosc_class.cpp:28-82, interaction :206-252 (the
+ * inline-projection branch; the other branch computes the same products),
+ * vacuum :260-277, out = I + ADV + KO + vacuum.  Written as the reference
+ * has them, including the nubar tau-tau row, whose vacuum term reads the nu
+ * fields (:269). */
+static void calrhs3(const orc_sim* s, double* const out[ORC_NVAR], double* const in[ORC_NVAR]) {
+    const int nx = s->nx, nz = s->nz, nv = s->nv;
+    const long snv = nv;
+    const long nzv = (long)nv * (nz + 2 * s->gz);
+    const double* vx = s->vx; const double* vy = s->vy;
+    const double* vz = s->vz; const double* vw = s->vw;
+    const double mu = s->mu, pmo = s->pmo;
+    const double hee = s->hv[0], hmm = s->hv[1], htt = s->hv[2], hemr = s->hv[3], hemi = s->hv[4];
+    const double hmtr = s->hv[5], hmti = s->hv[6], hter = s->hv[7], htei = s->hv[8];
+    const double ko_eps_x = -s->ko / s->dx / 16.0;
+    const double ko_eps_z = -s->ko / s->dz / 16.0;
+    const double ko5_eps_x = -s->ko / s->dx / 64.0;
+    const double ko5_eps_z = -s->ko / s->dz / 64.0;
+    const int dflt = s->ko_order != 5 && s->adv == ORC_ADV_CFD4;
+    enum { EE, MM, EMR, EMI, BEE, BMM, BEMR, BEMI, TT, MTR, MTI, TER, TEI, BTT, BMTR, BMTI, BTER, BTEI };
+
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < nz; ++j) {
+            /* 8 moment vectors: emRm emIp eemm mtRm mtIp teRm teIp mmtt */
+            double m[8][4];
+            memset(m, 0, sizeof m);
+            for (int k = 0; k < nv; ++k) {
+                const long p = orc_idx(s, i, j, k);
+                const double v4[4] = {1, vx[k], vy[k], vz[k]};
+                const double d[8] = {
+                    in[BEMR][p] - in[EMR][p],
+                    in[BEMI][p] + in[EMI][p],
+                    in[BEE][p] - in[BMM][p] - in[EE][p] + in[MM][p],
+                    in[BMTR][p] - in[MTR][p],
+                    in[BMTI][p] + in[MTI][p],
+                    in[BTER][p] - in[TER][p],
+                    in[BTEI][p] + in[TEI][p],
+                    in[BMM][p] - in[BTT][p] - in[MM][p] + in[TT][p]};
+                for (int q = 0; q < 4; ++q)
+                    for (int c = 0; c < 8; ++c) m[c][q] += v4[q] * vw[k] * d[c];
+            }
+            for (int v = 0; v < nv; ++v) {
+                const long p = orc_idx(s, i, j, v);
+                const double factor_x = -vx[v] / (12 * s->dx);
+                const double factor_z = -vz[v] / (12 * s->dz);
+                /* projections of the moment vectors onto v */
+                const double pA = INP(m[0]), pB = INP(m[1]), pC = INP(m[2]), pD = INP(m[3]);
+                const double pE = INP(m[4]), pF = INP(m[5]), pG = INP(m[6]), pH = INP(m[7]);
+                const double pTE = -(pC + pH);
+                double I[ORC_NVAR3], V[ORC_NVAR3];
+                /* nu (fl = 0) and nubar (fl = 1) rows; both see the same moments */
+                for (int fl = 0; fl < 2; ++fl) {
+                    const int oE = fl ? BEE : EE, oM = fl ? BMM : MM, oT = fl ? BTT : TT;
+                    const int oEMR = fl ? BEMR : EMR, oEMI = fl ? BEMI : EMI, oMTR = fl ? BMTR : MTR;
+                    const int oMTI = fl ? BMTI : MTI, oTER = fl ? BTER : TER, oTEI = fl ? BTEI : TEI;
+                    const double ee = in[oE][p], mm = in[oM][p], tt = in[oT][p];
+                    const double emr = in[oEMR][p], emi = in[oEMI][p], mtr = in[oMTR][p];
+                    const double mti = in[oMTI][p], ter = in[oTER][p], tei = in[oTEI][p];
+                    I[oE] = 2 * mu * (emi * pA + emr * pB - tei * pF - ter * pG);
+                    I[oM] = 2 * mu * (-emi * pA - emr * pB + mti * pD + mtr * pE);
+                    I[oT] = 2 * mu * (-mti * pD - mtr * pE + tei * pF + ter * pG);
+                    I[oEMR] = -mu * ((ee - mm) * pB + pC * emi - pE * ter + pD * tei - mti * pF + mtr * pG);
+                    I[oEMI] = mu * ((mm - ee) * pA + pC * emr - pE * tei - pD * ter + mti * pG + mtr * pF);
+                    I[oMTR] = -mu * ((mm - tt) * pE + pB * ter - pA * tei + emi * pF - emr * pG + mti * pH);
+                    I[oMTI] = mu * ((tt - mm) * pD + pB * tei + pA * ter - emi * pG - emr * pF + mtr * pH);
+                    I[oTER] = -mu * ((tt - ee) * pG - pB * mtr + pA * mti - emi * pD + emr * pE + tei * pTE);
+                    I[oTEI] = mu * ((ee - tt) * pF - pB * mti - pA * mtr + emi * pE + emr * pD + ter * pTE);
+                    V[oE] = -pmo * 2 * (emi * hemr + emr * hemi - htei * ter + hter * tei);
+                    V[oM] = pmo * 2 * (emi * hemr - emr * hemi + hmti * mtr - hmtr * mti);
+                    V[oEMR] = pmo * ((mm - ee) * hemi + emi * (hee - hmm) + hmti * ter + hmtr * tei - htei * mtr - hter * mti);
+                    V[oEMI] = pmo * ((ee - mm) * hemr - emr * (hee - hmm) - hmti * tei + hmtr * ter + htei * mti - hter * mtr);
+                    V[oTER] = pmo * ((ee - tt) * htei - emr * hmti - emi * hmtr + hemi * mtr + hemr * mti - tei * (hee - htt));
+                    V[oTEI] = pmo * ((tt - ee) * hter - emr * hmtr + emi * hmti - hemi * mti + hemr * mtr + ter * (hee - htt));
+                    if (fl == 0) {
+                        V[oMTR] = pmo * ((tt - mm) * hmti + emr * htei + emi * hter - hemi * ter - hemr * tei + mti * (hmm - htt));
+                        V[oMTI] = pmo * ((mm - tt) * hmtr + emr * hter - emi * htei + hemi * tei - hemr * ter - mtr * (hmm - htt));
+                    } else {   /* the same terms in the order of :270-271 */
+                        V[oMTR] = pmo * ((tt - mm) * hmti + emr * htei + emi * hter - hemr * tei - hemi * ter + mti * (hmm - htt));
+                        V[oMTI] = pmo * ((mm - tt) * hmtr + emr * hter - emi * htei - hemr * ter + hemi * tei - mtr * (hmm - htt));
+                    }
+                }
+                {   /* tau-tau rows (:268-269): both read the nu fields */
+                    const double mtr = in[MTR][p], mti = in[MTI][p], ter = in[TER][p], tei = in[TEI][p];
+                    V[TT] = -pmo * 2 * (hmti * mtr + hmtr * mti + htei * ter - hter * tei);
+                    V[BTT] = V[TT];
+                }
+                for (int f = 0; f < ORC_NVAR3; ++f) {
+                    const double* x = &in[f][p];
+                    /* the vacuum term enters as written: "- pmo*2*(...)" rows subtract */
+                    out[f][p] = I[f] + ADV_FD(x) + KO_FD(x) + V[f];
+                }
+            }
+        }
+}
+
 void orc_calrhs(const orc_sim* s, double* const out[ORC_NVAR], double* const in[ORC_NVAR]) {
+    if (s->nvar == ORC_NVAR3) {
+        calrhs3(s, out, in);
+        return;
+    }
     const int nx = s->nx, nz = s->nz, nv = s->nv;
     const long snv = nv;                                  /* z stride */
     const long nzv = (long)nv * (nz + 2 * s->gz);         /* x stride */
@@ -537,14 +713,6 @@ void orc_calrhs(const orc_sim* s, double* const out[ORC_NVAR], double* const in[
                 /* nuosc_class.cpp:180-183 */
                 const double factor_x = -vx[v] / (12 * s->dx);
                 const double factor_z = -vz[v] / (12 * s->dz);
-#define ADV_FD(x) (dflt ? (factor_z * (((x)[-2 * snv] - (x)[2 * snv]) - 8.0 * ((x)[-snv] - (x)[snv])) + \
-                          factor_x * (((x)[-2 * nzv] - (x)[2 * nzv]) - 8.0 * ((x)[-nzv] - (x)[nzv]))) \
-                       : (adv_axis(s->adv, (x), snv, vz[v], s->dz) + adv_axis(s->adv, (x), nzv, vx[v], s->dx)))
-#define KO_FD(x) (s->ko_order != 5 \
-                      ? (ko_eps_z * ((x)[-2 * snv] + (x)[2 * snv] - 4 * ((x)[-snv] + (x)[snv]) + 6 * (x)[0]) + \
-                         ko_eps_x * ((x)[-2 * nzv] + (x)[2 * nzv] - 4 * ((x)[-nzv] + (x)[nzv]) + 6 * (x)[0])) \
-                      : (ko5_axis((x), snv, ko5_eps_z) + ko5_axis((x), nzv, ko5_eps_x)))
-#define INP(a) ((a)[0] - (a)[1] * vx[v] - (a)[2] * vy[v] - (a)[3] * vz[v])
                 const double* ee = &in[0][ijv];
                 const double* mm = &in[1][ijv];
                 const double* emr = &in[2][ijv];
@@ -573,12 +741,12 @@ void orc_calrhs(const orc_sim* s, double* const out[ORC_NVAR], double* const in[
                 out[5][ijv] = -Ibee + ADV_FD(bmm) + KO_FD(bmm) + pmo * 2 * st * bemi[0];
                 out[6][ijv] = Ibemr + ADV_FD(bemr) + KO_FD(bemr) - pmo * 2 * ct * bemi[0];
                 out[7][ijv] = Ibemi + ADV_FD(bemi) + KO_FD(bemi) + pmo * (2 * ct * bemr[0] + st * (bee[0] - bmm[0]));
-#undef ADV_FD
-#undef KO_FD
-#undef INP
             }
         }
 }
+#undef ADV_FD
+#undef KO_FD
+#undef INP
 
 /* ------------------------------------------------------------------------- */
 /* RK4 (nuosc_class.cpp:312-351, 374-427)                                     */
@@ -590,7 +758,7 @@ static void vec1(orc_sim* s, double** v0, double** v1, double a, double** v2) {
         for (int j = 0; j < s->nz; ++j)
             for (int v = 0; v < s->nv; ++v) {
                 long k = orc_idx(s, i, j, v);
-                for (int f = 0; f < ORC_NVAR; ++f) v0[f][k] = v1[f][k] + a * v2[f][k];
+                for (int f = 0; f < s->nvar; ++f) v0[f][k] = v1[f][k] + a * v2[f][k];
             }
 }
 /* v0 = v1 + a*(v2 + v3) over the interior */
@@ -600,7 +768,7 @@ static void vec2(orc_sim* s, double** v0, double** v1, double a, double** v2, do
         for (int j = 0; j < s->nz; ++j)
             for (int v = 0; v < s->nv; ++v) {
                 long k = orc_idx(s, i, j, v);
-                for (int f = 0; f < ORC_NVAR; ++f) v0[f][k] = v1[f][k] + a * (v2[f][k] + v3[f][k]);
+                for (int f = 0; f < s->nvar; ++f) v0[f][k] = v1[f][k] + a * (v2[f][k] + v3[f][k]);
             }
 }
 /* v0 = a*v1 + b*(v2 + dt*v3) over the interior (nuosc_class.cpp:353-372) */
@@ -610,7 +778,7 @@ static void vec3(orc_sim* s, double** v0, double a, double** v1, double b, doubl
         for (int j = 0; j < s->nz; ++j)
             for (int v = 0; v < s->nv; ++v) {
                 long k = orc_idx(s, i, j, v);
-                for (int f = 0; f < ORC_NVAR; ++f) v0[f][k] = a * v1[f][k] + b * (v2[f][k] + dt * v3[f][k]);
+                for (int f = 0; f < s->nvar; ++f) v0[f][k] = a * v1[f][k] + b * (v2[f][k] + dt * v3[f][k]);
             }
 }
 static void swapbuf(double** a, double** b) {

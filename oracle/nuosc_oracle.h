@@ -20,7 +20,10 @@
  *                           ipt-0 pulse, outflow ghosts are quadratic
  *                           extrapolations of the three edge cells
  *   orc_calrhs              nuosc_class.cpp:11-307 (2 flavours, KO_ORD_3,
- *                           central 4th-order advection, vacuum on)
+ *                           central 4th-order advection, vacuum on; the
+ *                           discretisation variants of :112-183; 3 flavours,
+ *                           N_FLAVOR_3, :189-277 with the Hamiltonian of
+ *                           :543-569)
  *   orc_step_rk4            nuosc_class.cpp:312-351, 374-427
  *   orc_step_srk3           nuosc_class.cpp:353-372, 429-452 (with boundary
  *                           fills, which the reference's step_srk3 omits)
@@ -43,6 +46,15 @@
  *   - init: sum_v w G0 = 1, sum_v w G0b = alpha, trace and purity identities.
  *   - analysis: an independent numpy formulation of the diagnostics plus the
  *     pure-state/conservation identities.
+ *   - 3 flavours (the reference's N_FLAVOR_3 build stops at the preprocessor:
+ *     `#ifdef 0`, nuosc_class.cpp:207, so no reference run exists): with the
+ *     tau fields zero and pmo = 0 the 8 two-flavour rows equal the 2-flavour
+ *     RHS bit for bit and the 10 tau rows are zero; the interaction part
+ *     keeps Tr rho and Tr rho^2 of every direction (nu and nubar) for random
+ *     states; hv equals an independent U diag(0, dm21, dm31) U^T - Tr/3 in
+ *     the reference's angle convention.  The vacuum rows outside the e-mu
+ *     block are parity unpinned (they are not a commutator as written: see
+ *     orc_vacuum_h3).
  *   - open boundary (no reference counterpart): at mu=0 the exact solution is
  *     free streaming of the initial profile; the interior error converges at
  *     third order in dz (the order of the KO term) with inflow and outflow both
@@ -60,7 +72,12 @@
 extern "C" {
 #endif
 
-#define ORC_NVAR 8
+/* fields per state: 8 for 2 flavours (ee, mm, emr, emi, bee, bmm, bemr,
+ * bemi), 18 for 3 flavours (the same 8 first, then tt, mtr, mti, ter, tei,
+ * btt, bmtr, bmti, bter, btei); arrays of field pointers hold ORC_NVAR */
+#define ORC_NVAR 18
+#define ORC_NVAR2 8
+#define ORC_NVAR3 18
 
 typedef struct orc_sim {
     /* geometry (CartGrid.cpp:191-208) */
@@ -88,6 +105,12 @@ typedef struct orc_sim {
      * 5, and the advection scheme ORC_ADV_* */
     int ko_order;
     int adv;
+    /* flavours: nvar = 8 (2 flavours, default) or 18 (3 flavours, the
+     * reference's compile-time N_FLAVOR_3, common.h:10-16) */
+    int nvar;
+    /* 3-flavour vacuum Hamiltonian {hee, hmm, htt, hemr, hemi, hmtr, hmti,
+     * hter, htei} (nuosc_class.cpp:543-569) */
+    double hv[9];
 } orc_sim;
 
 #define ORC_ADV_CFD4 0      /* 4th-order central (default) */
@@ -118,6 +141,12 @@ int orc_set_boundary(orc_sim* s, int bc, double zc);
 int orc_set_scheme(orc_sim* s, int ko_order, int adv);
 /* stencil radius of a discretisation */
 int orc_scheme_radius(int ko_order, int adv);
+/* Number of flavours (2 or 3); 3 allocates the 10 extra fields (zero) and
+ * sets the vacuum Hamiltonian hv.  Call before orc_fill_init.  Returns 0, or
+ * -1 on bad input. */
+int orc_set_flavors(orc_sim* s, int nflavor);
+/* The 3-flavour vacuum Hamiltonian of nuosc_class.cpp:543-569 into hv[9]. */
+void orc_vacuum_h3(double hv[9]);
 
 /* nuosc_init.cpp:17-137.  ipt 0,2,3 as the reference; ipt 5 = seeded complex
  * perturbation (counter-based RNG keyed by seed and the interior index).
@@ -155,8 +184,8 @@ int orc_analysis(orc_sim* s, double out[9]);
  * returns the number of characters written to buf (buf >= 512 bytes). */
 int orc_format_analysis_line(orc_sim* s, const double out[9], char* buf, int buflen);
 
-/* Field access in the interior layout [nx][nz][nv] (f in 0..7 with the
- * reference's field order ee, mm, emr, emi, bee, bmm, bemr, bemi). */
+/* Field access in the interior layout [nx][nz][nv] (f in 0..nvar-1, the
+ * field order of ORC_NVAR). */
 void orc_get_field(const orc_sim* s, int which_buf, int f, double* dst);
 void orc_set_field(orc_sim* s, int which_buf, int f, const double* src);
 /* Whole ghost-inclusive buffer (lpts doubles). */

@@ -18,6 +18,9 @@ LIB_PATH = os.path.join(HERE, "liboracle.so")
 REF_VGRID = os.path.join(HERE, "_ref", "ref_vgrid")
 
 FIELDS = ("ee", "mm", "emr", "emi", "bee", "bmm", "bemr", "bemi")
+# 3 flavours: the 8 fields above, then these 10
+FIELDS3 = FIELDS + ("tt", "mtr", "mti", "ter", "tei", "btt", "bmtr", "bmti", "bter", "btei")
+NVAR_MAX = 18
 DBL_MAX = np.finfo(np.float64).max
 
 _lib = None
@@ -42,12 +45,13 @@ class _Sim(C.Structure):
         ("phy_time", C.c_double),
         ("n_nue0", C.c_double * 2),
         ("G0", C.POINTER(C.c_double)), ("G0b", C.POINTER(C.c_double)),
-        ("stat", C.POINTER(C.c_double) * 8), ("rhs", C.POINTER(C.c_double) * 8),
-        ("pre", C.POINTER(C.c_double) * 8), ("cor", C.POINTER(C.c_double) * 8),
+        ("stat", C.POINTER(C.c_double) * NVAR_MAX), ("rhs", C.POINTER(C.c_double) * NVAR_MAX),
+        ("pre", C.POINTER(C.c_double) * NVAR_MAX), ("cor", C.POINTER(C.c_double) * NVAR_MAX),
         ("analysis_lines", C.c_int),
         ("bc", C.c_int), ("zc", C.c_double), ("inflow_on", C.c_int),
         ("inflow_eps0", C.c_double), ("inflow_sigma", C.c_double),
         ("ko_order", C.c_int), ("adv", C.c_int),
+        ("nvar", C.c_int), ("hv", C.c_double * 9),
     ]
 
 
@@ -67,17 +71,20 @@ def lib():
         L.orc_fill_init.argtypes = [C.POINTER(_Sim), C.c_int] + [C.c_double] * 7 + [C.c_uint64]
         L.orc_fill_init.restype = C.c_int
         L.orc_seed_uniforms.argtypes = [C.c_uint64, C.c_long, dp, dp]
-        L.orc_periodic_ghosts.argtypes = [C.POINTER(_Sim), dp * 8]
+        L.orc_periodic_ghosts.argtypes = [C.POINTER(_Sim), dp * NVAR_MAX]
         L.orc_set_boundary.argtypes = [C.POINTER(_Sim), C.c_int, C.c_double]
         L.orc_set_boundary.restype = C.c_int
         L.orc_set_scheme.argtypes = [C.POINTER(_Sim), C.c_int, C.c_int]
         L.orc_set_scheme.restype = C.c_int
-        L.orc_open_ghosts.argtypes = [C.POINTER(_Sim), dp * 8, C.c_double]
-        L.orc_fill_ghosts.argtypes = [C.POINTER(_Sim), dp * 8, C.c_double]
-        L.orc_calrhs.argtypes = [C.POINTER(_Sim), dp * 8, dp * 8]
+        L.orc_set_flavors.argtypes = [C.POINTER(_Sim), C.c_int]
+        L.orc_set_flavors.restype = C.c_int
+        L.orc_vacuum_h3.argtypes = [dp]
+        L.orc_open_ghosts.argtypes = [C.POINTER(_Sim), dp * NVAR_MAX, C.c_double]
+        L.orc_fill_ghosts.argtypes = [C.POINTER(_Sim), dp * NVAR_MAX, C.c_double]
+        L.orc_calrhs.argtypes = [C.POINTER(_Sim), dp * NVAR_MAX, dp * NVAR_MAX]
         L.orc_step_rk4.argtypes = [C.POINTER(_Sim)]
         L.orc_step_srk3.argtypes = [C.POINTER(_Sim)]
-        L.orc_renormalize.argtypes = [C.POINTER(_Sim), dp * 8]
+        L.orc_renormalize.argtypes = [C.POINTER(_Sim), dp * NVAR_MAX]
         L.orc_analysis.argtypes = [C.POINTER(_Sim), dp]
         L.orc_analysis.restype = C.c_int
         L.orc_format_analysis_line.argtypes = [C.POINTER(_Sim), dp, C.c_char_p, C.c_int]
@@ -110,6 +117,14 @@ def seed_uniforms(seed: int, lin: int):
     return ur.value, ui.value
 
 
+def vacuum_h3():
+    """The oracle's 3-flavour vacuum Hamiltonian {hee, hmm, htt, hemr, hemi,
+    hmtr, hmti, hter, htei}."""
+    out = np.zeros(9)
+    lib().orc_vacuum_h3(_dp(out))
+    return out
+
+
 def ref_vgrid(nn: int):
     """Velocity grid printed by the reference's own icosahedron.h (oracle/_ref),
     or None when that binary was not built."""
@@ -130,7 +145,8 @@ class OracleSim:
     ADV_CFD4, ADV_CFD2, ADV_UPWIND2, ADV_LOPSIDED, ADV_MUSCL, ADV_OFF = range(6)
 
     def __init__(self, nn=4, gx=2, gz=2, x0=-3.2, x1=3.2, z0=-3.2, z1=3.2, dx=0.1, dz=0.1,
-                 cfl=0.4, ko=0.1, mu=1.0, pmo=0.0, renorm=0, vgrid_mode=0, bc=0, zc=0.0, ko_order=3, adv=0):
+                 cfl=0.4, ko=0.1, mu=1.0, pmo=0.0, renorm=0, vgrid_mode=0, bc=0, zc=0.0, ko_order=3, adv=0,
+                 nflavor=2):
         L = lib()
         p = L.orc_create(nn, gx, gz, x0, x1, z0, z1, dx, dz, cfl, ko, vgrid_mode)
         if not p:
@@ -143,6 +159,9 @@ class OracleSim:
         if L.orc_set_scheme(p, ko_order, adv) != 0:
             self.close()
             raise ValueError("orc_set_scheme rejected the configuration (ghost width below the stencil radius?)")
+        if L.orc_set_flavors(p, nflavor) != 0:
+            self.close()
+            raise ValueError("orc_set_flavors rejected the configuration")
 
     def close(self):
         if self._p:
@@ -158,6 +177,10 @@ class OracleSim:
     @property
     def s(self):
         return self._p.contents
+
+    @property
+    def nvar(self):
+        return self.s.nvar
 
     @property
     def shape(self):
@@ -211,8 +234,8 @@ class OracleSim:
 
     def get_state(self, buf=0):
         nx, nz, nv = self.shape
-        out = np.zeros((8, nx, nz, nv))
-        for f in range(8):
+        out = np.zeros((self.nvar, nx, nz, nv))
+        for f in range(self.nvar):
             tmp = np.zeros(nx * nz * nv)
             lib().orc_get_field(self._p, buf, f, _dp(tmp))
             out[f] = tmp.reshape(nx, nz, nv)
@@ -221,8 +244,8 @@ class OracleSim:
     def set_state(self, arr, buf=0):
         nx, nz, nv = self.shape
         arr = np.asarray(arr, dtype=np.float64)
-        assert arr.shape == (8, nx, nz, nv)
-        for f in range(8):
+        assert arr.shape == (self.nvar, nx, nz, nv)
+        for f in range(self.nvar):
             lib().orc_set_field(self._p, buf, f, _dp(np.ascontiguousarray(arr[f]).reshape(-1)))
 
     def raw_buffer(self, buf, f):
@@ -231,8 +254,8 @@ class OracleSim:
         return np.ctypeslib.as_array(ptr, (self.s.lpts,))
 
     def _ptrs(self, buf):
-        arr = (C.POINTER(C.c_double) * 8)()
-        for f in range(8):
+        arr = (C.POINTER(C.c_double) * NVAR_MAX)()
+        for f in range(self.nvar):
             arr[f] = lib().orc_field_ptr(self._p, buf, f)
         return arr
 
@@ -242,13 +265,13 @@ class OracleSim:
         lib().orc_fill_ghosts(self._p, self._ptrs(buf), self.phy_time if t is None else t)
 
     def ghosted(self, buf=0):
-        """(8, nx+2gx, nz+2gz, nv) view of a buffer including its ghost band."""
+        """(nvar, nx+2gx, nz+2gz, nv) view of a buffer including its ghost band."""
         s = self.s
         return np.stack([self.raw_buffer(buf, f).reshape(s.nx + 2 * s.gx, s.nz + 2 * s.gz, s.nv)
-                         for f in range(8)])
+                         for f in range(self.nvar)])
 
     def calrhs(self, state):
-        """RHS of an interior state (8,nx,nz,nv): boundary fill at the current
+        """RHS of an interior state (nvar,nx,nz,nv): boundary fill at the current
         time, then calRHS."""
         self.set_state(state, buf=self.BUF_PRE)
         self.fill_ghosts(self.BUF_PRE)
