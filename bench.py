@@ -20,6 +20,11 @@ Prints one JSON line:
                  in which the per-update targets are stated
   cpu_baseline   the CPU oracle on a 128 x 128 x 642 crop of the same physics,
                  timed on this host's cores (a reported baseline)
+
+--nflavor 3 runs the same box with 3 flavours (18 fields, pmo=1): every stage
+is a moment pass over its input plus the stage kernel, so the stage passes
+are (3, 5, 4, 4) of 18 fields; the roofline line is the stage-2 kernel alone
+(4 passes).
 """
 from __future__ import annotations
 
@@ -51,6 +56,7 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-autotune", action="store_true", help="box: keep the default z-segment plan")
     p.add_argument("--cpu-seconds", type=float, default=12.0)
+    p.add_argument("--nflavor", type=int, default=2, choices=(2, 3), help="box: 2 (8 fields) or 3 (18 fields)")
     p.add_argument("--workload", default="box", choices=("box", "batched", "served", "precision"),
                    help="box: one large run (default); batched: 64 uneven runs stepped together vs one by one; "
                         "served: Poisson arrivals into the run server; precision: the FP32 form against FP64 "
@@ -300,14 +306,15 @@ def bench_precision(a):
     print(json.dumps(out), flush=True)
 
 
-def cpu_baseline(nn, dz, seconds):
+def cpu_baseline(nn, dz, seconds, nflavor=2, pmo=0.0):
     """Oracle (OpenMP C restatement of the reference) on a 128x128 crop."""
     from oracle import oracle as O
     if not os.path.exists(O.LIB_PATH):
         O.build()
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
     half = 64 * dz
-    o = O.OracleSim(nn=nn, x0=-half, x1=half, z0=-half, z1=half, dx=dz, dz=dz, cfl=0.4, ko=0.1, mu=1.0, pmo=0.0)
+    o = O.OracleSim(nn=nn, x0=-half, x1=half, z0=-half, z1=half, dx=dz, dz=dz, cfl=0.4, ko=0.1, mu=1.0, pmo=pmo,
+                    nflavor=nflavor)
     o.fill_init(ipt=5, alpha=0.9, eps0=1e-6, sigma=1.0, lnue=0.6, lnueb=0.5, seed=12345)
     nx, nz, nv = o.shape
     o.step(1)  # warm caches / page in
@@ -324,7 +331,7 @@ def cpu_baseline(nn, dz, seconds):
         "unit": "site*dir RK4 updates/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{nx}x{nz}x{nv} crop of the same physics (FP64), {n} RK4 steps in {el:.1f} s, "
+        "sample": f"{nx}x{nz}x{nv} crop of the same physics ({nflavor} flavours, FP64), {n} RK4 steps in {el:.1f} s, "
                   f"OMP_NUM_THREADS={threads}",
     }
 
@@ -352,8 +359,13 @@ def main():
 
     prec = N.PREC_FP64 if a.precision == 64 else N.PREC_FP32
     esize = 8 if a.precision == 64 else 4
+    three = a.nflavor == 3
+    nvar = 18 if three else 8
+    # 3 flavours: the moment pass of each stage reads its input once more
+    passes = tuple(p + 1 for p in STAGE_PASSES) if three else STAGE_PASSES
+    pmo = 1.0 if three else 0.0
     cfg = dict(nv=a.nn, x0=-a.zmax, x1=a.zmax, z0=-a.zmax, z1=a.zmax, dx=a.dz, dz=a.dz, CFL=0.4, ko=0.1,
-               mu=1.0, pmo=0.0, precision=prec, use_graph=True)
+               mu=1.0, pmo=pmo, precision=prec, use_graph=True, nflavor=a.nflavor)
     g = N.NuOsc(device=0, **cfg)
     g.fillInitValue(ipt=5, alpha=0.9, eps0=1e-6, sigma=1.0, lnue=0.6, lnueb=0.5, seed=12345)
     nx, nz, nv = g.shape
@@ -375,9 +387,9 @@ def main():
     g.close()
 
     value = upd * a.steps / wall
-    s2_bytes = STAGE_PASSES[1] * 8 * esize * upd
+    s2_bytes = STAGE_PASSES[1] * nvar * esize * upd
     s2_gbs = s2_bytes / (stage_ms[1] * 1e-3) / 1e9
-    step_bytes = sum(STAGE_PASSES) * 8 * esize * upd
+    step_bytes = sum(passes) * nvar * esize * upd
     step_gbs = step_bytes / (wall / a.steps) / 1e9
     out = {
         "metric": "site*dir RK4 updates/s",
@@ -394,32 +406,33 @@ def main():
         "data": "synthetic (seeded eps=1e-6 complex perturbation)",
         "config": {
             "workload": f"{nx}x{nz} periodic box x {nv} icosahedral directions, "
-                        f"{'FP64' if a.precision == 64 else 'FP32 + FP64 moments'}, mu=1 ko=0.1 cfl=0.4 pmo=0",
+                        f"{'3 flavours (18 fields), ' if three else ''}"
+                        f"{'FP64' if a.precision == 64 else 'FP32 + FP64 moments'}, mu=1 ko=0.1 cfl=0.4 pmo={pmo:g}",
             "nx": nx, "nz": nz, "nv": nv, "dz": a.dz, "zlen": zlen,
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_stage stage 2",
+            "kernel": "k_stage3 stage 2" if three else "k_stage stage 2",
             "achieved": s2_gbs,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": s2_gbs / HBM_PEAK_GBS,
-            "traffic": traffic_from_profiles(a.precision),
+            "traffic": None if three else traffic_from_profiles(a.precision),
             "algorithmic_bytes_per_launch": s2_bytes,
             "launch_ms": stage_ms[1],
         },
         "step_roofline": {
             "achieved": step_gbs,
             "frac": step_gbs / HBM_PEAK_GBS,
-            "bytes_per_update": sum(STAGE_PASSES) * 8 * esize,
-            "vs_16pass_ceiling": value / (HBM_PEAK_GBS * 1e9 / (16 * 8 * esize)),
+            "bytes_per_update": sum(passes) * nvar * esize,
+            "vs_16pass_ceiling": value / (HBM_PEAK_GBS * 1e9 / (16 * nvar * esize)),
             "stage_ms": [round(x, 4) for x in stage_ms],
             "device_ms_per_step": dev_ms / a.steps,
         },
         "state_ok": ok,
     }
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(a.nn, a.dz, a.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(a.nn, a.dz, a.cpu_seconds, a.nflavor, pmo)
     print(json.dumps(out), flush=True)
 
 

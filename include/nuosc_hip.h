@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define NUOSC_ABI_VERSION 3
+#define NUOSC_ABI_VERSION 4
 
 /* status codes */
 #define NUOSC_OK 0
@@ -72,6 +72,9 @@ extern "C" {
 #define NUOSC_ADV_LOPSIDED 3  /* ADVEC_LOPESIDE: 4th-order lopsided */
 #define NUOSC_ADV_MUSCL 4     /* ADVEC_MUSCL: flux limited (superbee) */
 #define NUOSC_ADV_OFF 5       /* ADVEC_OFF */
+
+#define NUOSC_NVAR2 8         /* fields of a 2-flavour state */
+#define NUOSC_NVAR3 18        /* fields of a 3-flavour state */
 /* KO order 5 and the lopsided scheme have stencil radius 3: with the open z
  * boundary they need gz = 3.  Handles with a non-default discretisation are
  * not batchable. */
@@ -101,7 +104,22 @@ typedef struct nuosc_config {
                       * the reference's is 0 (nuosc_init.cpp:4, 98) */
     int ko_order;    /* Kreiss-Oliger order: 3 (or 0: the default) or 5 */
     int adv;         /* NUOSC_ADV_* (0: the default central scheme) */
+    int nflavor;     /* 2 (or 0: the default) or 3; see the field order below */
 } nuosc_config;
+
+/* Fields.  2 flavours (the reference's default build): 8 fields, index
+ * 0..7 = ee, mm, emr, emi, bee, bmm, bemr, bemi.  3 flavours (its
+ * compile-time N_FLAVOR_3, common.h:10-16; nuosc_class.h:17-67): 18 fields,
+ * the same 8 first, then 8..17 = tt, mtr, mti, ter, tei, btt, bmtr, bmti,
+ * bter, btei (the reference's getAllFields order is ee, mm, tt, emr, emi,
+ * mtr, mti, ter, tei, bee, bmm, btt, bemr, bemi, bmtr, bmti, bter, btei).
+ * The 3-flavour right-hand side is nuosc_class.cpp:28-82, 189-277 as
+ * written, with the vacuum Hamiltonian of :543-569 (mixing angles given in
+ * degrees/180 and used as radians, delta_cp = 0).  fillInitValue writes the
+ * 8 two-flavour fields and zeroes the tau fields; analysis() and
+ * renormalize() act on the two-flavour fields, as in the reference.
+ * 3-flavour handles take the default discretisation only and are not
+ * batchable. */
 
 /* Diagnostics of one analysis() call (nuosc_ana.cpp:34-137).  The first nine
  * are the analysis.dat columns; the rest are additions. */
@@ -179,7 +197,7 @@ int nuosc_synchronize(nuosc_sim* sim);
 /* NuOsc::calRHS(out, in) (nuosc_class.cpp:11-307) with the boundary fill of
  * its input (periodic: nuosc_boundary.cpp:4-52; open z: the NUOSC_BC_OPEN_Z
  * rule at the current phy_time): in/out are host arrays of
- * 8*nx*nz*nv doubles in the interior layout.  Test primitive; uses the
+ * nvar*nx*nz*nv doubles in the interior layout (nvar = 8 or 18 fields).  Test primitive; uses the
  * handle's scratch state buffers (the state itself is left untouched). */
 int nuosc_calrhs(nuosc_sim* sim, const double* in, double* out);
 
@@ -189,7 +207,8 @@ int nuosc_analysis(nuosc_sim* sim, nuosc_analysis_t* out);
 /* NuOsc::renormalize (nuosc_ana.cpp:140-166) on the current state. */
 int nuosc_renormalize(nuosc_sim* sim);
 
-/* State transfer (replaces direct access to v_stat->ee etc.).
+/* State transfer (replaces direct access to v_stat->ee etc.); field is a
+ * field index of the order above.
  * download_field: one field, interior layout (nx*nz*nv doubles).
  * download_field_ghosted: one field in the reference's ghosted layout with
  *   the ghost band filled by updatePeriodicBoundary's rule
@@ -224,7 +243,8 @@ int nuosc_profile_stages(nuosc_sim* sim, int nsteps, double* stage_ms);
  * rounds after a warm-up round, keeps each candidate's fastest time and
  * selects the best.  Call it at a step boundary: it overwrites only the RK4
  * scratch copies, never the state.  Results are cached per (nx, nz, nv,
- * precision) for the process. */
+ * precision) for the process.  3-flavour handles have no segment length:
+ * autotune returns NUOSC_OK with *best_zlen = 0 and set_zlen has no effect. */
 int nuosc_get_plan(const nuosc_sim* sim, int* zlen, int* waves);
 int nuosc_set_zlen(nuosc_sim* sim, int zlen);
 int nuosc_autotune(nuosc_sim* sim, int trials, int* best_zlen, double* best_ms);

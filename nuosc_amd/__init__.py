@@ -22,6 +22,8 @@ CLI_PATH = os.path.join(HERE, "bin", "nuosc")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "nuosc_hip.h")
 
 FIELDS = ("ee", "mm", "emr", "emi", "bee", "bmm", "bemr", "bemi")
+# 3 flavours: the 8 fields above, then these 10 (field indices 8..17)
+FIELDS3 = FIELDS + ("tt", "mtr", "mti", "ter", "tei", "btt", "bmtr", "bmti", "bter", "btei")
 DBL_MAX = np.finfo(np.float64).max
 
 NUOSC_OK, NUOSC_EINVAL, NUOSC_EHIP, NUOSC_ENOMEM, NUOSC_EUNSTABLE, NUOSC_ENODEV, NUOSC_EAGAIN = range(7)
@@ -46,7 +48,7 @@ class Config(C.Structure):
         ("cfl", C.c_double), ("ko", C.c_double), ("mu", C.c_double), ("pmo", C.c_double),
         ("renorm", C.c_int), ("vgrid", C.c_int), ("precision", C.c_int), ("bc", C.c_int),
         ("device", C.c_int), ("use_graph", C.c_int), ("zc", C.c_double),
-        ("ko_order", C.c_int), ("adv", C.c_int),
+        ("ko_order", C.c_int), ("adv", C.c_int), ("nflavor", C.c_int),
     ]
 
 
@@ -182,15 +184,18 @@ class NuOsc:
 
     def __init__(self, nv=4, gx=2, gz=2, x0=-1.0, x1=1.0, z0=-10.0, z1=10.0, dx=0.1, dz=0.1,
                  CFL=0.4, ko=0.0, mu=1.0, pmo=0.1, renorm=False, precision=PREC_FP64,
-                 vgrid_mode=VGRID_VORONOI, device=0, use_graph=True, bc=0, zc=0.0, ko_order=3, adv=ADV_CFD4):
+                 vgrid_mode=VGRID_VORONOI, device=0, use_graph=True, bc=0, zc=0.0, ko_order=3, adv=ADV_CFD4,
+                 nflavor=2):
         """bc: BC_PERIODIC (the reference's boundary) or BC_OPEN_Z (open z
         ends with the ipt-0 inflow pulse, x periodic); zc: centre of the ipt-0
         profile (0 in the reference); ko_order (3 or 5) and adv (ADV_*): the
-        discretisation (the reference's KO_ORD_3 / ADVEC_* build options)."""
+        discretisation (the reference's KO_ORD_3 / ADVEC_* build options);
+        nflavor 2 (8 fields, FIELDS) or 3 (18 fields, FIELDS3: the
+        reference's N_FLAVOR_3 build option)."""
         L = lib()
         cfg = Config(nn=nv, gx=gx, gz=gz, x0=x0, x1=x1, z0=z0, z1=z1, dx=dx, dz=dz, cfl=CFL, ko=ko,
                      mu=mu, pmo=pmo, renorm=int(renorm), vgrid=vgrid_mode, precision=precision, bc=bc,
-                     device=device, use_graph=int(use_graph), zc=zc, ko_order=ko_order, adv=adv)
+                     device=device, use_graph=int(use_graph), zc=zc, ko_order=ko_order, adv=adv, nflavor=nflavor)
         h = _vp()
         rc = L.nuosc_create(C.byref(cfg), C.byref(h))
         if rc != 0:
@@ -201,6 +206,7 @@ class NuOsc:
         L.nuosc_get_dims(h, C.byref(nx), C.byref(nz), C.byref(nvv), C.byref(lp))
         self.nx, self.nz, self.nv, self.lpts = nx.value, nz.value, nvv.value, lp.value
         self.gx, self.gz = gx, gz
+        self.nvar = 18 if nflavor == 3 else 8
 
     # -- plumbing ---------------------------------------------------------
     def _check(self, rc):
@@ -327,9 +333,9 @@ class NuOsc:
         return z.value, ms.value
 
     def calRHS(self, state):
-        """RHS of an interior state array (8, nx, nz, nv) -> (8, nx, nz, nv)."""
+        """RHS of an interior state array (nvar, nx, nz, nv) -> (nvar, nx, nz, nv)."""
         a = np.ascontiguousarray(state, dtype=np.float64)
-        assert a.shape == (8, self.nx, self.nz, self.nv)
+        assert a.shape == (self.nvar, self.nx, self.nz, self.nv)
         out = np.zeros_like(a)
         self._check(lib().nuosc_calrhs(self._h, _ptr(a), _ptr(out)))
         return out
@@ -351,7 +357,7 @@ class NuOsc:
         return out.reshape(self.nx, self.nz, self.nv)
 
     def get_state(self):
-        return np.stack([self.get_field(f) for f in range(8)])
+        return np.stack([self.get_field(f) for f in range(self.nvar)])
 
     def get_field_ghosted(self, f):
         shp = (self.nx + 2 * self.gx, self.nz + 2 * self.gz, self.nv)
@@ -361,8 +367,8 @@ class NuOsc:
 
     def set_state(self, state):
         a = np.asarray(state, dtype=np.float64)
-        assert a.shape == (8, self.nx, self.nz, self.nv)
-        for f in range(8):
+        assert a.shape == (self.nvar, self.nx, self.nz, self.nv)
+        for f in range(self.nvar):
             self._check(lib().nuosc_upload_field(self._h, f, _ptr(np.ascontiguousarray(a[f]).reshape(-1))))
         self._check(lib().nuosc_state_updated(self._h))
 
@@ -427,8 +433,9 @@ class Batch:
             pass
 
 
-def estimate_device_bytes(nv=4, x0=-1.0, x1=1.0, z0=-10.0, z1=10.0, dx=0.1, dz=0.1, precision=PREC_FP64, **_):
-    cfg = Config(nn=nv, gx=2, gz=2, x0=x0, x1=x1, z0=z0, z1=z1, dx=dx, dz=dz, precision=precision)
+def estimate_device_bytes(nv=4, x0=-1.0, x1=1.0, z0=-10.0, z1=10.0, dx=0.1, dz=0.1, precision=PREC_FP64, nflavor=2,
+                          **_):
+    cfg = Config(nn=nv, gx=2, gz=2, x0=x0, x1=x1, z0=z0, z1=z1, dx=dx, dz=dz, precision=precision, nflavor=nflavor)
     b = C.c_longlong()
     rc = lib().nuosc_estimate_device_bytes(C.byref(cfg), C.byref(b))
     if rc != 0:

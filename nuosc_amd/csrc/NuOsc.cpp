@@ -8,10 +8,12 @@
 #include <stdexcept>
 
 namespace {
-const char* kFieldNames[8] = {"ee", "mm", "emr", "emi", "bee", "bmm", "bemr", "bemi"};
+// field indices of include/nuosc_hip.h (2 flavours: the first 8)
+const char* kFieldNames[NUOSC_NVAR3] = {"ee",  "mm",  "emr", "emi",  "bee",  "bmm",  "bemr", "bemi", "tt",
+                                        "mtr", "mti", "ter", "tei",  "btt",  "bmtr", "bmti", "bter", "btei"};
 
-int field_index(const std::string& name) {
-    for (int f = 0; f < 8; ++f)
+int field_index(const std::string& name, int nvar) {
+    for (int f = 0; f < nvar; ++f)
         if (name == kFieldNames[f]) return f;
     if (name == "xx") return 1;  // the reference also calls mm "xx"
     return -1;
@@ -27,7 +29,7 @@ void NuOsc::check(int rc, const char* what) {
 
 NuOsc::NuOsc(int px, int pz, int nv, int nphi, int gx, int gz, double x0, double x1, double z0, double z1,
              double dx_, double dz_, double CFL_, double ko_, int precision, int vgrid, bool use_graph, int device,
-             int bc, double zc, int ko_order, int adv)
+             int bc, double zc, int ko_order, int adv, int nflavor)
     : dx(dx_), dz(dz_), CFL(CFL_), ko(ko_) {
     (void)nphi;  // polar velocity grid only
     if (px != 1 || pz != 1)
@@ -53,6 +55,8 @@ NuOsc::NuOsc(int px, int pz, int nv, int nphi, int gx, int gz, double x0, double
     c.zc = zc;
     c.ko_order = ko_order;
     c.adv = adv;
+    c.nflavor = nflavor;
+    nvar = nflavor == 3 ? NUOSC_NVAR3 : NUOSC_NVAR2;
     c.device = device;
     c.use_graph = use_graph ? 1 : 0;
     int rc = nuosc_create(&c, &sim_);
@@ -90,7 +94,7 @@ NuOsc::NuOsc(int px, int pz, int nv, int nphi, int gx, int gz, double x0, double
     std::printf("   Domain:  v: nv = %5d  nphi = %5d  on S2.\n", grid.nv, 0);
     std::printf("            x:( %12f %12f )  dx = %g\n", x0, x1, dx);
     std::printf("            z:( %12f %12f )  dz = %g\n", z0, z1, dz);
-    std::printf("   Fieldvar: 8 -- device memory %.2f GB\n", bytes / 1024. / 1024. / 1024.);
+    std::printf("   Fieldvar: %d -- device memory %.2f GB\n", nvar, bytes / 1024. / 1024. / 1024.);
     std::printf("   dt = %g     CFL = %g\n", dt, CFL);
     if (bc == NUOSC_BC_OPEN_Z)
         std::printf("   Boundary: x periodic, z open (ipt-0 inflow pulse centred at z = %g)\n", zc);
@@ -169,7 +173,7 @@ void NuOsc::step_srk3(int nsteps) {
 void NuOsc::synchronize() { check(nuosc_synchronize(sim_), "synchronize"); }
 
 void NuOsc::calRHS(std::vector<double>& out, const std::vector<double>& in) {
-    const size_t n = (size_t)8 * grid.nx * grid.nz * grid.nv;
+    const size_t n = (size_t)nvar * grid.nx * grid.nz * grid.nv;
     if (in.size() != n) throw std::runtime_error("calRHS: input size mismatch");
     out.assign(n, 0.0);
     check(nuosc_calrhs(sim_, in.data(), out.data()), "calRHS");
@@ -279,7 +283,7 @@ void NuOsc::checkSnapShot(int t) {
             if (var == "P3") {
                 check(nuosc_p3_slice(sim_, ss.x_slices.data(), sx, ss.v_slices.data(), sv, carr.data()), "P3 slice");
             } else {
-                const int f = field_index(var);
+                const int f = field_index(var, nvar);
                 if (f < 0) throw std::runtime_error("unknown snapshot variable " + var);
                 const std::vector<double> fld = field(f);
                 for (int a = 0; a < sx; ++a)

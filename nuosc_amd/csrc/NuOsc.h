@@ -12,7 +12,8 @@
 // and read the device state at the time of the dump.
 // Added: precision, velocity-grid weights, boundary (periodic or open z
 // with an inflowing ipt-0 pulse centred at zc), the discretisation (the
-// reference's KO_ORD_3 / ADVEC_* build options: ko_order, adv) and, beside analysis.dat
+// reference's KO_ORD_3 / ADVEC_* / N_FLAVOR_3 build options: ko_order, adv,
+// nflavor) and, beside analysis.dat
 // (whose columns stay the reference's), growth.dat with |rho_ex| per
 // analysis call for linear growth rates.
 #pragma once
@@ -38,7 +39,7 @@ struct NuOscGrid {
 };
 
 struct NuOscSnapShot {
-    std::vector<std::string> vars;   // "P3", or field names ee, mm, emr, ...
+    std::vector<std::string> vars;   // "P3", or field names ee, mm, emr, ... (3 flavours: also tt, mtr, ...)
     std::string fntpl;
     int every = 1;
     std::vector<int> x_slices, v_slices;
@@ -58,11 +59,12 @@ public:
     double pmo = 0.1;
     double mu = 1.0;
     bool renorm = false;
+    int nvar = NUOSC_NVAR2;   // fields: 8 (2 flavours) or 18 (3 flavours, N_FLAVOR_3)
 
     NuOsc(int px, int pz, int nv, int nphi, int gx, int gz, double x0, double x1, double z0, double z1, double dx,
           double dz, double CFL, double ko, int precision = NUOSC_PREC_FP64, int vgrid = NUOSC_VGRID_VORONOI,
           bool use_graph = true, int device = 0, int bc = NUOSC_BC_PERIODIC, double zc = 0.0, int ko_order = 3,
-          int adv = NUOSC_ADV_CFD4);
+          int adv = NUOSC_ADV_CFD4, int nflavor = 2);
     ~NuOsc();
     NuOsc(const NuOsc&) = delete;
     NuOsc& operator=(const NuOsc&) = delete;
@@ -80,7 +82,7 @@ public:
     void step_srk3();
     void step_srk3(int nsteps);
     void synchronize();
-    // out/in: 8 fields in the interior layout [nx][nz][nv]
+    // out/in: nvar fields in the interior layout [nx][nz][nv]
     void calRHS(std::vector<double>& out, const std::vector<double>& in);
     // prints the reference's stdout line and appends to analysis.dat;
     // returns the full diagnostics

@@ -29,6 +29,7 @@ struct nuosc_sim {
     nuosc_config cfg{};
     // geometry (CartGrid.cpp:191-208)
     int nx = 0, nz = 0, nv = 0, nvp = 0, nch = 0;
+    int nf = kNVar;                 // fields per site record: 8 (2 flavours) or 18 (3 flavours)
     long long srec = 0, xrow = 0;   // site-record and x-row strides of the state copies (elements)
     long long state_elems = 0;      // elements allocated for the four copies
     double dt = 0, ds_L = 0, ct = 0, st = 0;
@@ -44,7 +45,7 @@ struct nuosc_sim {
     void* st_ya = nullptr;
     void* st_yb = nullptr;
     double* mom_part = nullptr;  // partial records of the last stage output
-    double* mom_u = nullptr;     // totals of u, yA, yB
+    double* mom_u = nullptr;     // totals of u, yA, yB (3 flavours: mom_u alone, the stage input's moments)
     double* mom_a = nullptr;
     double* mom_b = nullptr;
     double* d_vgrid = nullptr;  // vx | vy | vz | vw
@@ -88,22 +89,23 @@ int fail(nuosc_sim* s, int code, const std::string& msg) {
 bool is64(const nuosc_sim* s) { return s->cfg.precision == NUOSC_PREC_FP64; }
 
 // Strides of the four state copies (separate allocations, site records of
-// 8 nvp elements); ok = false: the stage kernel's per-wave range of x rows
-// or the moment records do not fit 32-bit buffer offsets.
+// nf nvp elements); ok = false: the 2-flavour stage kernel's per-wave range
+// of x rows or the moment records do not fit 32-bit buffer offsets (the
+// 3-flavour kernels address with 64-bit pointers).
 struct CopyLayout {
     long long srec, xrow, elems;   // strides and elements allocated for all four copies
     bool ok;
 };
-CopyLayout copy_layout(int nx, int nz, int nzs, int nvp, int nch, int esize) {
+CopyLayout copy_layout(int nx, int nz, int nzs, int nvp, int nch, int esize, int nf) {
     Geometry gq{};
     gq.nx = nx;
     gq.nz = nz;
     gq.nch = nch;
     gq.nvp = nvp;
     gq.nxs = nx + 2 * kXg + 1;
-    gq.srec = (long long)kNVar * nvp;
+    gq.srec = (long long)nf * nvp;
     gq.xrow = (long long)nzs * gq.srec;
-    if (!buffer_addressable(gq, esize)) return CopyLayout{0, 0, 0, false};
+    if (nf == kNVar && !buffer_addressable(gq, esize)) return CopyLayout{0, 0, 0, false};
     return CopyLayout{gq.srec, gq.xrow, 4 * copy_elems(gq), true};
 }
 
@@ -133,11 +135,107 @@ int ensure_tmp(nuosc_sim* s, size_t bytes) {
     return NUOSC_OK;
 }
 
+// 3-flavour vacuum Hamiltonian (nuosc_class.cpp:543-569), as written there:
+// splittings times 1267*2, mixing angles degrees/180 used as radians,
+// delta_cp = 0, minus a third of its trace on the diagonal.  Order: hee, hmm,
+// htt, hemr, hemi, hmtr, hmti, hter, htei (the reference's names).
+void vacuum_h3(double h[9]) {
+    const double m21 = 7.39e-5 * 1267. * 2., m31 = 2.5229e-3 * 1267. * 2.;
+    const double t12 = 33.82 / 180., t13 = 8.61 / 180., t23 = 48.3 / 180., cp = 0.;
+    const double c12 = std::cos(t12), s12 = std::sin(t12), c13 = std::cos(t13), s13 = std::sin(t13);
+    const double c23 = std::cos(t23), s23 = std::sin(t23);
+    // elements of the mixing matrix used (real and imaginary parts)
+    const double e2 = s12 * c13, e3r = s13 * std::cos(cp), e3i = -s13 * std::sin(cp);
+    const double m2r = c12 * c23 - s12 * s13 * s23 * std::cos(cp), m2i = -s12 * s13 * s23 * std::sin(cp);
+    const double u2r = -c12 * s23 - s12 * s13 * c23 * std::cos(cp), u2i = -s12 * s13 * c23 * std::sin(cp);
+    const double m3 = c13 * s23, u3 = c13 * c23;
+    const double dee = m21 * (e2 * e2) + m31 * (s13 * s13);
+    const double dmm = m21 * (m2r * m2r + m2i * m2i) + m31 * (m3 * m3);
+    const double dtt = m21 * (u2r * u2r + u2i * u2i) + m31 * (u3 * u3);
+    h[3] = m21 * (e2 * m2r) + m31 * (e3r * m3);
+    h[4] = m21 * (-e2 * m2i) + m31 * (e3i * m3);
+    h[5] = m21 * (e2 * u2r) + m31 * (e3r * u3);
+    h[6] = m21 * (-e2 * u2i) + m31 * (e3i * u3);
+    h[7] = m21 * (m2r * u2r + m2i * u2i) + m31 * (m3 * u3);
+    h[8] = m21 * (u2r * m2i - m2r * u2i);
+    const double tr = (dee + dmm + dtt) / 3.0;
+    h[0] = dee - tr;
+    h[1] = dmm - tr;
+    h[2] = dtt - tr;
+}
+
+// ghost rows of a stage input at stage time t + cdt: open z ends (if any),
+// then the periodic x ghost rows (whole stored rows, z ghosts included)
+template <typename T>
+hipError_t stage_ghosts(nuosc_sim* s, T* y, double cdt) {
+    if (s->geo.bc == NUOSC_BC_OPEN_Z) {
+        const hipError_t e =
+            launch_open_ghosts<T>(s->geo, y, s->d_g0, s->d_g0 + s->nv, s->inflow, s->d_time, cdt, s->stream);
+        if (e != hipSuccess) return e;
+    }
+    return launch_xghosts<T>(s->geo, y, s->stream);
+}
+
+// 3 flavours: each stage is the ghost fill of its input, the moments of the
+// input, then the stage kernel (the RK4 combination of enqueue_step).  Events
+// ev[2k], ev[2k+1] bracket the stage-(k+1) kernel alone.
+template <typename T>
+int enqueue_step3(nuosc_sim* s, hipEvent_t* ev) {
+    const double dt = s->dt, c6 = 1.0 / 6.0 * dt, c2 = 0.5 * dt;
+    T* u = static_cast<T*>(s->st_u);
+    T* w = static_cast<T*>(s->st_w);
+    T* ya = static_cast<T*>(s->st_ya);
+    T* yb = static_cast<T*>(s->st_yb);
+    const Geometry& g = s->geo;
+    double* M = s->mom_u;
+    auto mark = [&](int k) { return ev ? hipEventRecord(ev[k], s->stream) : hipSuccess; };
+    struct St { int stage; T* yin; T* uu; T* ww; T* yout; double cw, cy, ct; };
+    const St st[4] = {{1, u, nullptr, nullptr, ya, 0.0, c2, 0.0},
+                      {2, ya, u, w, yb, dt, c2, c2},
+                      {3, yb, u, nullptr, ya, 0.0, dt, c2},
+                      {4, ya, nullptr, w, u, c6, 0.0, dt}};
+    for (int k = 0; k < 4; ++k) {
+        HIPCK(s, stage_ghosts<T>(s, st[k].yin, st[k].ct));
+        HIPCK(s, launch_moments3<T>(g, st[k].yin, M, s->stream));
+        HIPCK(s, mark(2 * k));
+        HIPCK(s, launch_stage3<T>(st[k].stage, g, s->ph, st[k].yin, st[k].uu, st[k].ww, st[k].yout, M, st[k].cw,
+                                  st[k].cy, s->stream));
+        HIPCK(s, mark(2 * k + 1));
+    }
+    if (g.bc == NUOSC_BC_OPEN_Z) HIPCK(s, launch_advance_time(s->d_time, dt, s->stream));
+    if (s->cfg.renorm) HIPCK(s, launch_renorm<T>(g, u, s->d_g0, s->d_g0 + s->nv, s->stream));
+    return NUOSC_OK;
+}
+
+// 3 flavours, SSP-RK3 (the stages of enqueue_step_srk3)
+template <typename T>
+int enqueue_step3_srk3(nuosc_sim* s) {
+    const double dt = s->dt;
+    T* u = static_cast<T*>(s->st_u);
+    T* ya = static_cast<T*>(s->st_ya);
+    T* yb = static_cast<T*>(s->st_yb);
+    const Geometry& g = s->geo;
+    double* M = s->mom_u;
+    HIPCK(s, stage_ghosts<T>(s, u, 0.0));
+    HIPCK(s, launch_moments3<T>(g, u, M, s->stream));
+    HIPCK(s, launch_stage3<T>(5, g, s->ph, u, nullptr, nullptr, ya, M, 0.0, dt, s->stream));
+    HIPCK(s, stage_ghosts<T>(s, ya, dt));
+    HIPCK(s, launch_moments3<T>(g, ya, M, s->stream));
+    HIPCK(s, launch_stage3<T>(6, g, s->ph, ya, u, nullptr, yb, M, 0.0, dt, s->stream, 3.0 / 4.0, 1.0 / 4.0));
+    HIPCK(s, stage_ghosts<T>(s, yb, 0.5 * dt));
+    HIPCK(s, launch_moments3<T>(g, yb, M, s->stream));
+    HIPCK(s, launch_stage3<T>(7, g, s->ph, yb, u, nullptr, u, M, 0.0, dt, s->stream, 1.0 / 3.0, 2.0 / 3.0));
+    if (g.bc == NUOSC_BC_OPEN_Z) HIPCK(s, launch_advance_time(s->d_time, dt, s->stream));
+    if (s->cfg.renorm) HIPCK(s, launch_renorm<T>(g, u, s->d_g0, s->d_g0 + s->nv, s->stream));
+    return NUOSC_OK;
+}
+
 // One RK4 step: the four stage launches (+ optional renormalisation).  With
 // ev != nullptr events ev[2k], ev[2k+1] bracket the stage-(k+1) kernel alone
 // (not its boundary fill or moment-total pass).
 template <typename T>
 int enqueue_step(nuosc_sim* s, hipEvent_t* ev = nullptr) {
+    if (s->nf == kNVar3) return enqueue_step3<T>(s, ev);
     const double dt = s->dt;
     const double c6 = 1.0 / 6.0 * dt, c2 = 0.5 * dt;
     T* u = static_cast<T*>(s->st_u);
@@ -194,6 +292,7 @@ int enqueue_step(nuosc_sim* s, hipEvent_t* ev = nullptr) {
 // each stage input with its boundary fill (stage times t, t+dt, t+dt/2).
 template <typename T>
 int enqueue_step_srk3(nuosc_sim* s) {
+    if (s->nf == kNVar3) return enqueue_step3_srk3<T>(s);
     const double dt = s->dt;
     T* u = static_cast<T*>(s->st_u);
     T* ya = static_cast<T*>(s->st_ya);
@@ -230,8 +329,13 @@ int enqueue_step_srk3(nuosc_sim* s) {
     return NUOSC_OK;
 }
 
-// partial records and totals of the state u
+// partial records and totals of the state u (3 flavours: nothing to keep,
+// every stage computes the moments of its input)
 int refresh_moments(nuosc_sim* s) {
+    if (s->nf == kNVar3) {
+        s->have_moments = true;
+        return NUOSC_OK;
+    }
     if (is64(s)) HIPCK(s, launch_moments<double>(s->geo, (double*)s->st_u, s->mom_part, s->stream));
     else HIPCK(s, launch_moments<float>(s->geo, (float*)s->st_u, s->mom_part, s->stream));
     HIPCK(s, launch_moment_totals(s->geo, !is64(s), s->mom_part, s->mom_u, s->stream));
@@ -276,7 +380,7 @@ int set_device_time(nuosc_sim* s) {
 }
 
 int check_field(nuosc_sim* s, int f) {
-    if (f < 0 || f >= kNVar) return fail(s, NUOSC_EINVAL, "field index out of range");
+    if (f < 0 || f >= s->nf) return fail(s, NUOSC_EINVAL, "field index out of range");
     return NUOSC_OK;
 }
 
@@ -317,7 +421,7 @@ int nuosc_abi_version(void) { return NUOSC_ABI_VERSION; }
 
 const char* nuosc_build_info(void) {
     return "libnuosc_hip: gfx950 RK4 stage kernels (fp64, fp32 + fp64 moments; periodic and open-z boundaries), "
-           "abi 3";
+           "abi 4";
 }
 
 int nuosc_vgrid(int nn, int mode, double* vx, double* vy, double* vz, double* vw) {
@@ -355,6 +459,10 @@ int nuosc_create(const nuosc_config* cfg, nuosc_sim** out) {
         return fail(nullptr, NUOSC_EINVAL, "this discretisation has stencil radius 3: the open z boundary needs gz = 3");
     if (c.vgrid != NUOSC_VGRID_VORONOI && c.vgrid != NUOSC_VGRID_EQUAL)
         return fail(nullptr, NUOSC_EINVAL, "unknown velocity grid");
+    if (c.nflavor != 0 && c.nflavor != 2 && c.nflavor != 3)
+        return fail(nullptr, NUOSC_EINVAL, "nflavor must be 2 or 3");
+    if (c.nflavor == 3 && (c.ko_order == 5 || c.adv != NUOSC_ADV_CFD4))
+        return fail(nullptr, NUOSC_EINVAL, "3 flavours run with the default discretisation (KO 3, central advection)");
 
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(nullptr, NUOSC_ENODEV, "no HIP device");
@@ -362,6 +470,7 @@ int nuosc_create(const nuosc_config* cfg, nuosc_sim** out) {
 
     nuosc_sim* s = new nuosc_sim();
     s->cfg = c;
+    s->nf = (c.nflavor == 3) ? kNVar3 : kNVar;
     // CartGrid.cpp:197-201
     s->nx = int((c.x1 - c.x0) / c.dx);
     s->nz = int((c.z1 - c.z0) / c.dz);
@@ -387,7 +496,7 @@ int nuosc_create(const nuosc_config* cfg, nuosc_sim** out) {
     const int lpr = dirs_per_chunk((int)s->esize);
     s->nch = (s->nv + lpr - 1) / lpr;
     {
-        const CopyLayout L = copy_layout(s->nx, s->nz, nzs, s->nvp, s->nch, (int)s->esize);
+        const CopyLayout L = copy_layout(s->nx, s->nz, nzs, s->nvp, s->nch, (int)s->esize, s->nf);
         if (!L.ok) {
             delete s;
             return fail(nullptr, NUOSC_EINVAL,
@@ -404,7 +513,8 @@ int nuosc_create(const nuosc_config* cfg, nuosc_sim** out) {
     const double theta = 37 * M_PI / 180.;
     s->ct = std::cos(2 * theta);
     s->st = std::sin(2 * theta);
-    s->ph = Physics{c.mu, c.pmo, s->ct, s->st};
+    s->ph = Physics{c.mu, c.pmo, s->ct, s->st, {}};
+    vacuum_h3(s->ph.h3);
 
     auto bail = [&](int code, const std::string& m) {
         std::string msg = m;
@@ -418,8 +528,9 @@ int nuosc_create(const nuosc_config* cfg, nuosc_sim** out) {
         return bail(NUOSC_EHIP, "event creation failed");
 
     const size_t state_bytes = (size_t)s->state_elems * s->esize;
+    const bool three = s->nf == kNVar3;
     const size_t part_bytes = (size_t)(s->nx + 1) * s->nz * s->nch * kMomRec * sizeof(double);
-    const size_t mom_bytes = (size_t)s->nx * s->nz * kMomStride * sizeof(double);
+    const size_t mom_bytes = (size_t)s->nx * s->nz * (three ? kNMom3 : kMomStride) * sizeof(double);
     void** states[4] = {&s->st_u, &s->st_w, &s->st_ya, &s->st_yb};
     {
         const size_t copy_bytes = state_bytes / 4;
@@ -431,6 +542,7 @@ int nuosc_create(const nuosc_config* cfg, nuosc_sim** out) {
     }
     double** moms[4] = {&s->mom_part, &s->mom_u, &s->mom_a, &s->mom_b};
     for (double** p : moms) {
+        if (three && p != &s->mom_u) continue;   // 3 flavours: one record of the stage input's moments
         const size_t b = (p == &s->mom_part) ? part_bytes : mom_bytes;
         if (hipMalloc((void**)p, b) != hipSuccess) return bail(NUOSC_ENOMEM, "moment allocation failed");
         s->device_bytes += (long long)b;
@@ -466,6 +578,7 @@ int nuosc_create(const nuosc_config* cfg, nuosc_sim** out) {
     s->geo.bc = c.bc;
     s->geo.nzs = nzs;
     s->geo.zoff = zoff;
+    s->geo.nf = s->nf;
     s->inflow = Inflow{0, 0.0, 1.0, c.zc, c.z0, c.dz};
     if (c.bc == NUOSC_BC_OPEN_Z) {
         if (hipMalloc((void**)&s->d_time, sizeof(double)) != hipSuccess) return bail(NUOSC_ENOMEM, "allocation failed");
@@ -783,6 +896,11 @@ std::map<std::tuple<int, int, int, int>, int> g_tune_cache;
 
 int nuosc_autotune(nuosc_sim* s, int trials, int* best_zlen, double* best_ms) {
     if (!s || trials < 1) return NUOSC_EINVAL;
+    if (s->nf == kNVar3) {   // the 3-flavour kernels have no z-segment length to choose
+        if (best_zlen) *best_zlen = 0;
+        if (best_ms) *best_ms = -1;
+        return NUOSC_OK;
+    }
     const auto key = std::make_tuple(s->nx, s->nz, s->nv, s->cfg.precision);
     {
         std::lock_guard<std::mutex> g(g_tune_mu);
@@ -853,14 +971,24 @@ int nuosc_calrhs(nuosc_sim* s, const double* in, double* out) {
     int rc = ensure_tmp(s, fbytes);
     if (rc) return rc;
     const size_t fe = (size_t)s->nx * s->nz * s->nv;
-    for (int f = 0; f < kNVar; ++f) {
+    for (int f = 0; f < s->nf; ++f) {
         HIPCK(s, hipMemcpyAsync(s->d_tmp, in + f * fe, fbytes, hipMemcpyHostToDevice, s->stream));
         if (is64(s)) HIPCK(s, launch_unpack<double>(s->geo, s->d_tmp, (double*)s->st_ya, f, s->stream));
         else HIPCK(s, launch_unpack<float>(s->geo, s->d_tmp, (float*)s->st_ya, f, s->stream));
     }
     rc = fill_state_ghosts(s, s->st_ya);
     if (rc) return rc;
-    if (is64(s)) {
+    if (s->nf == kNVar3) {
+        if (is64(s)) {
+            HIPCK(s, launch_moments3<double>(s->geo, (double*)s->st_ya, s->mom_u, s->stream));
+            HIPCK(s, launch_stage3<double>(0, s->geo, s->ph, (double*)s->st_ya, nullptr, nullptr, (double*)s->st_yb,
+                                           s->mom_u, 0.0, 0.0, s->stream));
+        } else {
+            HIPCK(s, launch_moments3<float>(s->geo, (float*)s->st_ya, s->mom_u, s->stream));
+            HIPCK(s, launch_stage3<float>(0, s->geo, s->ph, (float*)s->st_ya, nullptr, nullptr, (float*)s->st_yb,
+                                          s->mom_u, 0.0, 0.0, s->stream));
+        }
+    } else if (is64(s)) {
         HIPCK(s, launch_moments<double>(s->geo, (double*)s->st_ya, s->mom_part, s->stream));
         HIPCK(s, launch_moment_totals(s->geo, false, s->mom_part, s->mom_a, s->stream));
         HIPCK(s, launch_stage<double>(0, s->geo, s->ph, s->plan, (double*)s->st_ya, nullptr, nullptr,
@@ -871,7 +999,7 @@ int nuosc_calrhs(nuosc_sim* s, const double* in, double* out) {
         HIPCK(s, launch_stage<float>(0, s->geo, s->ph, s->plan, (float*)s->st_ya, nullptr, nullptr,
                                      (float*)s->st_yb, s->mom_a, nullptr, 0.0, 0.0, s->stream));
     }
-    for (int f = 0; f < kNVar; ++f) {
+    for (int f = 0; f < s->nf; ++f) {
         if (is64(s)) HIPCK(s, launch_pack<double>(s->geo, (double*)s->st_yb, f, s->d_tmp, s->stream));
         else HIPCK(s, launch_pack<float>(s->geo, (float*)s->st_yb, f, s->d_tmp, s->stream));
         HIPCK(s, hipMemcpyAsync(out + f * fe, s->d_tmp, fbytes, hipMemcpyDeviceToHost, s->stream));
@@ -1107,6 +1235,7 @@ int nuosc_batch_add(nuosc_batch* b, nuosc_sim* s) {
     if (s->cfg.bc != NUOSC_BC_PERIODIC) return bfail(b, NUOSC_EINVAL, "only periodic runs can be batched");
     if (s->geo.ko5 || s->geo.adv != NUOSC_ADV_CFD4)
         return bfail(b, NUOSC_EINVAL, "only runs with the default discretisation can be batched");
+    if (s->nf != kNVar) return bfail(b, NUOSC_EINVAL, "only 2-flavour runs can be batched");
     for (nuosc_sim* m : b->members)
         if (m == s) return bfail(b, NUOSC_EINVAL, "already a member");
     b->members.push_back(s);
@@ -1220,10 +1349,12 @@ int nuosc_estimate_device_bytes(const nuosc_config* c, long long* bytes) {
     const long long align = 128 / esize, nvp = (nv + align - 1) / align * align;
     const long long lpr = dirs_per_chunk(esize), nch = (nv + lpr - 1) / lpr;
     const long long nzs = nz + (c->bc == NUOSC_BC_OPEN_Z ? 2 * c->gz : 0);
-    const CopyLayout L = copy_layout((int)nx, (int)nz, (int)nzs, (int)nvp, (int)nch, esize);
+    const int nf = c->nflavor == 3 ? kNVar3 : kNVar;
+    const CopyLayout L = copy_layout((int)nx, (int)nz, (int)nzs, (int)nvp, (int)nch, esize, nf);
     if (!L.ok) return NUOSC_EINVAL;
     long long b = L.elems * esize;                                     // u, w, yA, yB
-    b += (nx + 1) * nz * nch * kMomRec * 8 + 3ll * nx * nz * kMomStride * 8;  // moments
+    if (nf == kNVar3) b += nx * nz * kNMom3 * 8;                       // moments
+    else b += (nx + 1) * nz * nch * kMomRec * 8 + 3ll * nx * nz * kMomStride * 8;
     b += 4 * nv * 8 + 2 * nv * 8 + 2048ll * kNAna * 16 + kNAna * 8;    // v-grid, G0, analysis
     b += 2 * nz * 8;                                                   // init tables
     *bytes = b;

@@ -2,12 +2,15 @@
 // and the HIP kernels (nuosc_kernels.hip).  Not part of the public ABI.
 //
 // Device data layout (state copies u, w, yA, yB):
-//   site records of 8 fields (ee, mm, emr, emi, bee, bmm, bemr, bemi), each
-//   field a row of nvp directions (v fastest; nvp = nv rounded up so that
-//   every field row starts on a 128-byte boundary):
+//   site records of nf fields, each field a row of nvp directions (v
+//   fastest; nvp = nv rounded up so that every field row starts on a 128-byte
+//   boundary):
 //     copy[xs][zs][f][nvp],  xs = i + kXg (stored x row), zs = j + zoff
-//   so the 8 fields of a site are one contiguous block (srec = 8 nvp elements
+//   so the fields of a site are one contiguous block (srec = nf nvp elements
 //   between site records) and a stored x row (nzs records) is contiguous.
+//   2 flavours: nf = 8 (ee, mm, emr, emi, bee, bmm, bemr, bemi); 3 flavours:
+//   nf = 18, the same 8 first, then tt, mtr, mti, ter, tei, btt, bmtr, bmti,
+//   bter, btei.
 //   Each copy is its own allocation.  Stored x rows: kXg = 3 periodic
 //   ghost rows at each end (copies of interior rows nx-3 .. nx-1 and
 //   0 .. 2, refreshed before every stage from the interior) and one spare row at
@@ -33,7 +36,9 @@
 
 namespace nuosc {
 
-constexpr int kNVar = 8;
+constexpr int kNVar = 8;         // fields of the 2-flavour state
+constexpr int kNVar3 = 18;       // fields of the 3-flavour state
+constexpr int kNMom3 = 32;       // 3-flavour moment doubles per site (8 vectors of 4)
 constexpr int kWave = 64;
 constexpr int kMomStride = 16;   // doubles per per-site moment total record (12 used)
 constexpr int kMomRec = 12;      // doubles per (site, chunk) partial moment record
@@ -57,7 +62,7 @@ struct Geometry {
     int wpb;               // wavefronts per stage-kernel block (waves_per_block)
     int nch;               // direction chunks per site = ceil(nv / lpr)
     int nxs;               // stored x rows: nx + 2 kXg + 1 (ghost rows and the spare row)
-    long long srec;        // elements between consecutive site records of one copy (8 nvp)
+    long long srec;        // elements between consecutive site records of one copy (nf nvp)
     long long xrow;        // elements between stored x rows of one copy: nzs * srec
     double twelve_dx, twelve_dz;   // 12*dx, 12*dz (advection denominators)
     double ko_x, ko_z;     // -ko/dx/16, -ko/dz/16
@@ -67,7 +72,23 @@ struct Geometry {
     const double *vx, *vy, *vz, *vw;   // device, length nv
     int bc;                // 0 periodic, 1 open in z
     int nzs, zoff;         // stored z rows per x row, stored row of interior z = 0
+    int nf;                // fields per site record: 8 (2 flavours) or 18 (3 flavours)
 };
+
+// k mod n in 0 .. n-1
+__host__ __device__ inline int wrapi(int k, int n) {
+    k %= n;
+    return k < 0 ? k + n : k;
+}
+
+// Blocks are dealt to the 8 XCDs round-robin (block b runs on XCD b % 8).
+// Return the logical block index so that each XCD works on one contiguous
+// range of logical blocks (a bijection for any grid size).
+__host__ __device__ inline int xcd_remap(int b, int nb) {
+    const int per = nb / 8, rem = nb % 8;
+    const int x = b % 8, loc = b / 8;
+    return (x < rem) ? x * (per + 1) + loc : rem * (per + 1) + (x - rem) * per + loc;
+}
 
 // element offset of field f, stored z row js, direction 0 of interior x row i
 // (i = -kXg .. nx + kXg; nx + kXg is the spare row)
@@ -90,6 +111,9 @@ struct Inflow {
 
 struct Physics {
     double mu, pmo, ct, st;
+    // 3-flavour vacuum Hamiltonian {hee, hmm, htt, hemr, hemi, hmtr, hmti,
+    // hter, htei} (nuosc_class.cpp:543-569)
+    double h3[9];
 };
 
 // One RK4 stage over the whole grid (k = L(yin)).
@@ -216,6 +240,19 @@ hipError_t launch_open_ghosts(const Geometry& g, T* y, const double* G0, const d
                               const double* t_dev, double cdt, hipStream_t s);
 // *t_dev += dt (device copy of the physical time, read by the boundary fill)
 hipError_t launch_advance_time(double* t_dev, double dt, hipStream_t s);
+
+// Three flavours (nf = 18; the reference's default discretisation, KO 3 and
+// central 4th-order advection).  Moments: per site 32 doubles, the vectors
+// {emRm, emIp, eemm, mtRm, mtIp, teRm, teIp, mmtt} x (1, vx, vy, vz)
+// (nuosc_class.cpp:28-82), summed over all directions of y in a fixed order.
+template <typename T>
+hipError_t launch_moments3(const Geometry& g, const T* y, double* mom, hipStream_t s);
+// One stage over the whole grid with the moments mom of yin; the stage
+// numbers and coefficients of launch_stage.
+template <typename T>
+hipError_t launch_stage3(int stage, const Geometry& g, const Physics& ph, const T* yin, const T* u, T* w,
+                         T* yout, const double* mom, double cw, double cy, hipStream_t s, double ca = 0.0,
+                         double cb = 0.0);
 
 // Renormalisation of the state to |P| = 1 (the reference's formula).
 template <typename T>

@@ -1,0 +1,292 @@
+// nuosc_flavor3.hip -- CDNA4 (gfx950) kernels of the 3-flavour time stepper
+// (the reference's compile-time N_FLAVOR_3: 18 fields, nuosc_class.cpp:28-82,
+// 189-277; vacuum Hamiltonian :543-569).
+//
+// Every stage is two launches over the state copies (the layout of
+// nuosc_dev.h with nf = 18):
+//   k_moments3  one wavefront per site: the 32 velocity moments of the stage
+//               input, summed per lane over directions lane, lane+64, ... and
+//               then across the wave by halving exchanges (a fixed order, so a
+//               run's result does not depend on the launch);
+//   k_stage3    one wavefront per (site, 64 directions), four x rows per
+//               block: the site's 18 centre values and 8 moment projections
+//               give the interaction and vacuum terms; each field's 8 stencil
+//               neighbours are then read straight from HBM/L2 and the stage
+//               combination of the RK4 or SSP-RK3 stage is stored.
+// The logical block order walks x quads inside z tiles of 16 sites, so the
+// z neighbours of a block were read a block or two earlier and its x
+// neighbours (the next quads) 16 blocks earlier, on the same XCD: the stencil
+// re-reads hit L2, and the HBM traffic stays close to one read of the stage
+// input.
+#include <hip/hip_runtime.h>
+
+#include "nuosc_dev.h"
+
+namespace nuosc {
+namespace {
+
+constexpr int kF3Rows = 4;    // x rows (wavefronts) per stage block
+constexpr int kF3Tile = 16;   // z sites per tile of the logical block order
+
+enum F3Field { EE, MM, EMR, EMI, BEE, BMM, BEMR, BEMI, TT, MTR, MTI, TER, TEI, BTT, BMTR, BMTI, BTER, BTEI };
+
+typedef const __attribute__((address_space(4))) double cdouble3;
+
+// Keeps half of M values and adds the partner's other half (lane ^ d): the
+// lanes with bit d set keep the upper half.
+template <int M>
+__device__ __forceinline__ void halve_xor(const double* in, double* out, int d, bool hi) {
+#pragma unroll
+    for (int q = 0; q < M / 2; ++q) {
+        const double keep = hi ? in[q + M / 2] : in[q];
+        const double send = hi ? in[q] : in[q + M / 2];
+        out[q] = keep + __shfl_xor(send, d);
+    }
+}
+
+// Wave total of 32 per-lane values: lane L ends with the total of value L >> 1.
+__device__ __forceinline__ double wave_sum32(const double (&a)[32], int lane) {
+    double b[16], c[8], d[4], e[2], f[1];
+    halve_xor<32>(a, b, 32, lane & 32);
+    halve_xor<16>(b, c, 16, lane & 16);
+    halve_xor<8>(c, d, 8, lane & 8);
+    halve_xor<4>(d, e, 4, lane & 4);
+    halve_xor<2>(e, f, 2, lane & 2);
+    return f[0] + __shfl_xor(f[0], 1);
+}
+
+// nuosc_class.cpp:57-71: moment differences of one point
+template <typename T>
+__device__ __forceinline__ void moment_diffs3(const T (&y)[kNVar3], double (&d)[8]) {
+    d[0] = (double)y[BEMR] - (double)y[EMR];
+    d[1] = (double)y[BEMI] + (double)y[EMI];
+    d[2] = (((double)y[BEE] - (double)y[BMM]) - (double)y[EE]) + (double)y[MM];
+    d[3] = (double)y[BMTR] - (double)y[MTR];
+    d[4] = (double)y[BMTI] + (double)y[MTI];
+    d[5] = (double)y[BTER] - (double)y[TER];
+    d[6] = (double)y[BTEI] + (double)y[TEI];
+    d[7] = (((double)y[BMM] - (double)y[BTT]) - (double)y[MM]) + (double)y[TT];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_moments3(const Geometry g, const T* __restrict__ y,
+                                                  double* __restrict__ mom) {
+    const int lane = threadIdx.x & 63;
+    const long long site = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (site >= (long long)g.nx * g.nz) return;   // whole wavefronts
+    double acc[32];
+#pragma unroll
+    for (int q = 0; q < 32; ++q) acc[q] = 0.0;
+    const T* p = y + site_off(g, site);
+    for (int v = lane; v < g.nv; v += 64) {
+        T x[kNVar3];
+#pragma unroll
+        for (int f = 0; f < kNVar3; ++f) x[f] = p[(long long)f * g.nvp + v];
+        const double w = g.vw[v];
+        const double q4[4] = {w, g.vx[v] * w, g.vy[v] * w, g.vz[v] * w};
+        double d[8];
+        moment_diffs3<T>(x, d);
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+#pragma unroll
+            for (int s = 0; s < 4; ++s) acc[4 * c + s] += q4[s] * d[c];
+    }
+    const double tot = wave_sum32(acc, lane);
+    if ((lane & 1) == 0) mom[site * kNMom3 + (lane >> 1)] = tot;
+}
+
+template <typename T>
+struct F3Args {
+    Geometry g;
+    const T* yin;
+    const T* u;
+    T* w;
+    T* yout;
+    const double* mom;
+    double cw, cy, ca, cb;
+    double mu, pmo;
+    double h[9];
+    int tiles, quads;
+};
+
+// Interaction (nuosc_class.cpp:206-252) and vacuum (:260-277) terms of one
+// flavour block (nu: o = 0, nubar: o = 1) from the centre values c and the
+// moment projections P = {emRm, emIp, eemm, mtRm, mtIp, teRm, teIp, mmtt}.
+template <typename T>
+__device__ __forceinline__ void local_terms3(const T (&c)[kNVar3], const T (&P)[8], T mu, T pmo, const T (&h)[9],
+                                             T (&I)[kNVar3], T (&V)[kNVar3]) {
+    const T hee = h[0], hmm = h[1], htt = h[2], hemr = h[3], hemi = h[4], hmtr = h[5], hmti = h[6], hter = h[7],
+            htei = h[8];
+    const T pA = P[0], pB = P[1], pC = P[2], pD = P[3], pE = P[4], pF = P[5], pG = P[6], pH = P[7];
+    const T pTE = -(pC + pH);
+    const T mu2 = (T)2 * mu, pmo2 = pmo * (T)2;
+#pragma unroll
+    for (int fl = 0; fl < 2; ++fl) {
+        const int oE = fl ? BEE : EE, oM = fl ? BMM : MM, oT = fl ? BTT : TT, oEMR = fl ? BEMR : EMR;
+        const int oEMI = fl ? BEMI : EMI, oMTR = fl ? BMTR : MTR, oMTI = fl ? BMTI : MTI, oTER = fl ? BTER : TER;
+        const int oTEI = fl ? BTEI : TEI;
+        const T ee = c[oE], mm = c[oM], tt = c[oT], emr = c[oEMR], emi = c[oEMI], mtr = c[oMTR], mti = c[oMTI],
+                ter = c[oTER], tei = c[oTEI];
+        I[oE] = mu2 * (emi * pA + emr * pB - tei * pF - ter * pG);
+        I[oM] = mu2 * (-emi * pA - emr * pB + mti * pD + mtr * pE);
+        I[oT] = mu2 * (-mti * pD - mtr * pE + tei * pF + ter * pG);
+        I[oEMR] = -mu * ((ee - mm) * pB + pC * emi - pE * ter + pD * tei - mti * pF + mtr * pG);
+        I[oEMI] = mu * ((mm - ee) * pA + pC * emr - pE * tei - pD * ter + mti * pG + mtr * pF);
+        I[oMTR] = -mu * ((mm - tt) * pE + pB * ter - pA * tei + emi * pF - emr * pG + mti * pH);
+        I[oMTI] = mu * ((tt - mm) * pD + pB * tei + pA * ter - emi * pG - emr * pF + mtr * pH);
+        I[oTER] = -mu * ((tt - ee) * pG - pB * mtr + pA * mti - emi * pD + emr * pE + tei * pTE);
+        I[oTEI] = mu * ((ee - tt) * pF - pB * mti - pA * mtr + emi * pE + emr * pD + ter * pTE);
+        V[oE] = -(pmo2 * (emi * hemr + emr * hemi - htei * ter + hter * tei));
+        V[oM] = pmo2 * (emi * hemr - emr * hemi + hmti * mtr - hmtr * mti);
+        V[oEMR] = pmo * ((mm - ee) * hemi + emi * (hee - hmm) + hmti * ter + hmtr * tei - htei * mtr - hter * mti);
+        V[oEMI] = pmo * ((ee - mm) * hemr - emr * (hee - hmm) - hmti * tei + hmtr * ter + htei * mti - hter * mtr);
+        V[oTER] = pmo * ((ee - tt) * htei - emr * hmti - emi * hmtr + hemi * mtr + hemr * mti - tei * (hee - htt));
+        V[oTEI] = pmo * ((tt - ee) * hter - emr * hmtr + emi * hmti - hemi * mti + hemr * mtr + ter * (hee - htt));
+        if (fl == 0) {
+            V[oMTR] = pmo * ((tt - mm) * hmti + emr * htei + emi * hter - hemi * ter - hemr * tei + mti * (hmm - htt));
+            V[oMTI] = pmo * ((mm - tt) * hmtr + emr * hter - emi * htei + hemi * tei - hemr * ter - mtr * (hmm - htt));
+        } else {   // the same terms in the order of nuosc_class.cpp:270-271
+            V[oMTR] = pmo * ((tt - mm) * hmti + emr * htei + emi * hter - hemr * tei - hemi * ter + mti * (hmm - htt));
+            V[oMTI] = pmo * ((mm - tt) * hmtr + emr * hter - emi * htei - hemr * ter + hemi * tei - mtr * (hmm - htt));
+        }
+    }
+    // both tau-tau rows read the nu fields (nuosc_class.cpp:268-269)
+    V[TT] = -(pmo2 * (hmti * c[MTR] + hmtr * c[MTI] + htei * c[TER] - hter * c[TEI]));
+    V[BTT] = V[TT];
+}
+
+template <typename T, int STAGE>
+__global__ __launch_bounds__(256) void k_stage3(const F3Args<T> a) {
+    const Geometry& g = a.g;
+    const int lb = xcd_remap(blockIdx.x, gridDim.x);
+    // logical block -> (direction chunk, z tile, x quad, z site in the tile)
+    const int jj = lb % kF3Tile;
+    int r = lb / kF3Tile;
+    const int qx = r % a.quads;
+    r /= a.quads;
+    const int jt = r % a.tiles;
+    const int ch = r / a.tiles;
+    const int j = jt * kF3Tile + jj;
+    const int i = __builtin_amdgcn_readfirstlane(qx * kF3Rows + (int)(threadIdx.x >> 6));
+    const int v = ch * 64 + (int)(threadIdx.x & 63);
+    if (j >= g.nz || i >= g.nx || v >= g.nv) return;
+
+    const long long o = elem_off(g, i, j + g.zoff, 0) + v;
+    long long zo[4];   // z - 2, z - 1, z + 1, z + 2: stored ghost rows (open z) or the periodic wrap
+    {
+        const int ks[4] = {-2, -1, 1, 2};
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            zo[q] = (long long)(g.bc ? ks[q] : wrapi(j + ks[q], g.nz) - j) * g.srec;
+    }
+    const long long xo = g.xrow;
+
+    // centre values and moment projections
+    T c[kNVar3];
+#pragma unroll
+    for (int f = 0; f < kNVar3; ++f) c[f] = a.yin[o + (long long)f * g.nvp];
+    const double vxd = g.vx[v], vyd = g.vy[v], vzd = g.vz[v];
+    cdouble3* m = (cdouble3*)(a.mom + ((long long)i * g.nz + j) * kNMom3);
+    T P[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q)   // INP, nuosc_class.cpp:187
+        P[q] = (T)(m[4 * q] - m[4 * q + 1] * vxd - m[4 * q + 2] * vyd - m[4 * q + 3] * vzd);
+    T h[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) h[q] = (T)a.h[q];
+    T I[kNVar3], V[kNVar3];
+    local_terms3<T>(c, P, (T)a.mu, (T)a.pmo, h, I, V);
+
+    // advection and Kreiss-Oliger factors (nuosc_class.cpp:120-123, 180-183)
+    const T fx = (T)(-vxd / g.twelve_dx), fz = (T)(-vzd / g.twelve_dz);
+    const T kx = (T)g.ko_x, kz = (T)g.ko_z;
+    const T cw = (T)a.cw, cy = (T)a.cy, ca = (T)a.ca, cb = (T)a.cb, third = (T)(1.0 / 3.0);
+#pragma unroll
+    for (int f = 0; f < kNVar3; ++f) {
+        const T* p = a.yin + o + (long long)f * g.nvp;
+        const T zm2 = p[zo[0]], zm1 = p[zo[1]], zp1 = p[zo[2]], zp2 = p[zo[3]];
+        const T xm2 = p[-2 * xo], xm1 = p[-xo], xp1 = p[xo], xp2 = p[2 * xo];
+        const T c0 = c[f];
+        const T adv = fz * ((zm2 - zp2) - (T)8.0 * (zm1 - zp1)) + fx * ((xm2 - xp2) - (T)8.0 * (xm1 - xp1));
+        const T ko = kz * (zm2 + zp2 - (T)4 * (zm1 + zp1) + (T)6 * c0) + kx * (xm2 + xp2 - (T)4 * (xm1 + xp1) + (T)6 * c0);
+        const T k = I[f] + adv + ko + V[f];
+        const long long of = o + (long long)f * g.nvp;
+        if constexpr (STAGE == 0) {
+            a.yout[of] = k;
+        } else if constexpr (STAGE == 1 || STAGE == 5) {   // Y2 = u + dt/2 k1; SSP-RK3 u1 = u + dt k
+            a.yout[of] = c0 + cy * k;
+        } else if constexpr (STAGE == 2) {                 // A = Y2 + u + dt k2, Y3 = u + dt/2 k2
+            const T uc = a.u[of];
+            a.w[of] = (c0 + uc) + cw * k;
+            a.yout[of] = uc + cy * k;
+        } else if constexpr (STAGE == 3) {                 // Y4 = u + dt k3
+            a.yout[of] = a.u[of] + cy * k;
+        } else if constexpr (STAGE == 4) {                 // u' = (A + Y4) / 3 + dt/6 k4
+            a.yout[of] = (a.w[of] + c0) * third + cw * k;
+        } else {                                           // SSP-RK3 (nuosc_class.cpp:440, 444, 369)
+            a.yout[of] = ca * a.u[of] + cb * (c0 + cy * k);
+        }
+    }
+}
+
+template <typename T, int STAGE>
+hipError_t launch3(const F3Args<T>& a, unsigned nb, hipStream_t s) {
+    hipLaunchKernelGGL((k_stage3<T, STAGE>), dim3(nb), dim3(64 * kF3Rows), 0, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+template <typename T>
+hipError_t launch_moments3(const Geometry& g, const T* y, double* mom, hipStream_t s) {
+    const long long sites = (long long)g.nx * g.nz;
+    if (g.nf != kNVar3 || sites <= 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_moments3<T>), dim3((unsigned)((sites + 3) / 4)), dim3(256), 0, s, g, y, mom);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_stage3(int stage, const Geometry& g, const Physics& ph, const T* yin, const T* u, T* w, T* yout,
+                         const double* mom, double cw, double cy, hipStream_t s, double ca, double cb) {
+    if (g.nf != kNVar3) return hipErrorInvalidValue;
+    F3Args<T> a{};
+    a.g = g;
+    a.yin = yin;
+    a.u = u;
+    a.w = w;
+    a.yout = yout;
+    a.mom = mom;
+    a.cw = cw;
+    a.cy = cy;
+    a.ca = ca;
+    a.cb = cb;
+    a.mu = ph.mu;
+    a.pmo = ph.pmo;
+    for (int q = 0; q < 9; ++q) a.h[q] = ph.h3[q];
+    a.tiles = (g.nz + kF3Tile - 1) / kF3Tile;
+    a.quads = (g.nx + kF3Rows - 1) / kF3Rows;
+    const long long nch = (g.nv + 63) / 64;
+    const long long nb = nch * a.tiles * a.quads * kF3Tile;
+    if (nb <= 0 || nb > 0x7fffffffLL) return hipErrorInvalidValue;
+    switch (stage) {
+        case 0: return launch3<T, 0>(a, (unsigned)nb, s);
+        case 1: return launch3<T, 1>(a, (unsigned)nb, s);
+        case 2: return launch3<T, 2>(a, (unsigned)nb, s);
+        case 3: return launch3<T, 3>(a, (unsigned)nb, s);
+        case 4: return launch3<T, 4>(a, (unsigned)nb, s);
+        case 5: return launch3<T, 5>(a, (unsigned)nb, s);
+        case 6: return launch3<T, 6>(a, (unsigned)nb, s);
+        case 7: return launch3<T, 7>(a, (unsigned)nb, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template hipError_t launch_moments3<double>(const Geometry&, const double*, double*, hipStream_t);
+template hipError_t launch_moments3<float>(const Geometry&, const float*, double*, hipStream_t);
+template hipError_t launch_stage3<double>(int, const Geometry&, const Physics&, const double*, const double*,
+                                          double*, double*, const double*, double, double, hipStream_t, double,
+                                          double);
+template hipError_t launch_stage3<float>(int, const Geometry&, const Physics&, const float*, const float*, float*,
+                                         float*, const double*, double, double, hipStream_t, double, double);
+
+}  // namespace nuosc

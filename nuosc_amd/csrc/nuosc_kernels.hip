@@ -65,19 +65,6 @@ constexpr int kVals = StageShape<T>::V;
 // ---------------------------------------------------------------------------
 // helpers
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int wrapi(int k, int n) {
-    k %= n;
-    return k < 0 ? k + n : k;
-}
-
-// Blocks are dealt to the 8 XCDs round-robin (block b runs on XCD b % 8).
-// Return the logical block index so that each XCD works on one contiguous
-// range of logical blocks (a bijection for any grid size).
-__device__ __forceinline__ int xcd_remap(int b, int nb) {
-    const int per = nb / 8, rem = nb % 8;
-    const int x = b % 8, loc = b / 8;
-    return (x < rem) ? x * (per + 1) + loc : rem * (per + 1) + (x - rem) * per + loc;
-}
 
 // Orders the LDS accesses of a wavefront: everything before is visible to
 // every lane of the wave after (the wave executes in lockstep; this keeps the
@@ -881,7 +868,7 @@ constexpr int kXgPerThread = 4;
 XgArgs xg_args(const Geometry& g, const void* y, int esize) {
     XgArgs a;
     a.y = (unsigned long long)y;
-    a.recw = kNVar * g.nvp * esize / 16;
+    a.recw = (int)(g.srec * esize / 16);
     a.recstride = (int)(g.srec * esize / 16);
     a.words = (long long)g.nzs * a.recw;
     a.rowstride = g.xrow * esize / 16;
@@ -1198,10 +1185,12 @@ __global__ __launch_bounds__(256) void k_init(const Geometry g, int ipt, const d
     const int v = (int)(n % g.nvp);
     const long long o = site_off(g, site) + v;
     if (v >= g.nv) {  // row padding
-#pragma unroll
-        for (int f = 0; f < kNVar; ++f) y[o + f * g.nvp] = (T)0;
+        for (int f = 0; f < g.nf; ++f) y[o + f * g.nvp] = (T)0;
         return;
     }
+    // 3 flavours: the tau fields keep their construction value, zero (the
+    // reference's fillInitValue writes the 8 two-flavour fields only)
+    for (int f = kNVar; f < g.nf; ++f) y[o + f * g.nvp] = (T)0;
     const int j = (int)(site % g.nz);
     double tmpr, tmpi = 0.0, p3o;
     if (ipt == 5) {
@@ -1496,7 +1485,7 @@ hipError_t launch_renorm(const Geometry& g, T* y, const double* G0, const double
 // open z boundary (no reference counterpart: updateInjetOpenBoundary,
 // nuosc_boundary.cpp:54-57, is a stub)
 // ---------------------------------------------------------------------------
-// One thread per (x row, side, ghost depth, direction) writes all 8 fields.
+// One thread per (x row, side, ghost depth, direction) writes all fields.
 template <typename T>
 __global__ __launch_bounds__(256) void k_open_ghosts(const Geometry g, T* __restrict__ y,
                                                      const double* __restrict__ G0,
@@ -1534,6 +1523,7 @@ __global__ __launch_bounds__(256) void k_open_ghosts(const Geometry g, T* __rest
         y[o + 5 * g.nvp] = (T)(hgb * (1.0 - p3o));
         y[o + 6 * g.nvp] = (T)(hgb * a);
         y[o + 7 * g.nvp] = (T)0.0;
+        for (int f = kNVar; f < g.nf; ++f) y[o + f * g.nvp] = (T)0.0;
     } else {
         // quadratic extrapolation through the three edge cells
         const double w0 = q == 1 ? 3.0 : (q == 2 ? 6.0 : 10.0);
@@ -1542,8 +1532,7 @@ __global__ __launch_bounds__(256) void k_open_ghosts(const Geometry g, T* __rest
         const int e0 = side == 0 ? 0 : g.nz - 1, dir = side == 0 ? 1 : -1;
         const long long c0 = elem_off(g, i, e0 + g.zoff, 0) + v;
         const long long step = (long long)dir * g.srec;   // one site record along z
-#pragma unroll
-        for (int f = 0; f < kNVar; ++f) {
+        for (int f = 0; f < g.nf; ++f) {
             const T* p = y + c0 + f * g.nvp;
             y[o + f * g.nvp] = (T)(w0 * (double)p[0] + w1 * (double)p[step] + w2 * (double)p[2 * step]);
         }

@@ -45,7 +45,7 @@ int main(int argc, char* argv[]) {
     int precision = NUOSC_PREC_FP64, vgrid = NUOSC_VGRID_VORONOI, snap_p3 = 0, use_graph = 1, device = 0;
     int bc = NUOSC_BC_PERIODIC, srk3 = 0;
     double zc = 0.0;
-    int ko_order = 3, adv = NUOSC_ADV_CFD4;
+    int ko_order = 3, adv = NUOSC_ADV_CFD4, nflavor = 2;
     bool gz_set = false;
     unsigned long long seed = 12345;
 
@@ -107,6 +107,7 @@ int main(int argc, char* argv[]) {
         else if (!std::strcmp(a, "--zc")) zc = std::atof(val());
         else if (!std::strcmp(a, "--srk3")) srk3 = std::atoi(val());
         else if (!std::strcmp(a, "--ko_order")) ko_order = std::atoi(val());
+        else if (!std::strcmp(a, "--nflavor")) nflavor = std::atoi(val());
         else if (!std::strcmp(a, "--adv")) {
             const char* m = val();
             static const char* const names[] = {"cfd4", "cfd2", "upwind2", "lopsided", "muscl", "off"};
@@ -126,7 +127,7 @@ int main(int argc, char* argv[]) {
     if (!gz_set && (ko_order == 5 || adv == NUOSC_ADV_LOPSIDED)) gx = gz = 3;
     try {
         NuOsc state(px, pz, nv_in, nphi, gx, gz, x0, x1, z0, z1, dx, dz, cfl, ko, precision, vgrid, use_graph != 0,
-                    device, bc, zc, ko_order, adv);
+                    device, bc, zc, ko_order, adv, nflavor);
         const long long lpts = state.grid.get_lpts();
         state.set_mu(mu);
         state.set_pmo(pmo);
