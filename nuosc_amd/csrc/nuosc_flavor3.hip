@@ -15,9 +15,11 @@
 //               combination of the RK4 or SSP-RK3 stage is stored.
 // The logical block order walks x quads inside z tiles of 16 sites, so the
 // z neighbours of a block were read a block or two earlier and its x
-// neighbours (the next quads) 16 blocks earlier, on the same XCD: the stencil
-// re-reads hit L2, and the HBM traffic stays close to one read of the stage
-// input.
+// neighbours (the next quads) 16 blocks earlier, on the same XCD, and most
+// stencil re-reads hit L2 (HBM reads 1.3-1.6x one pass of the stage input at
+// 512 x 512 x 642).  The cost is the 8 neighbour loads per field and lane:
+// with 18 fields a register z-window like the 2-flavour kernel's would take
+// 180 VGPRs in FP64.
 #include <hip/hip_runtime.h>
 
 #include "nuosc_dev.h"
@@ -155,8 +157,11 @@ __device__ __forceinline__ void local_terms3(const T (&c)[kNVar3], const T (&P)[
     V[BTT] = V[TT];
 }
 
+// The stage input is never one of the stage's outputs (yin differs from u, w
+// and yout in every stage): declared restrict, the stencil loads of later
+// fields can be issued before the stores of earlier ones.
 template <typename T, int STAGE>
-__global__ __launch_bounds__(256) void k_stage3(const F3Args<T> a) {
+__device__ __forceinline__ void stage3_body(const F3Args<T>& a, const T* __restrict__ yin) {
     const Geometry& g = a.g;
     const int lb = xcd_remap(blockIdx.x, gridDim.x);
     // logical block -> (direction chunk, z tile, x quad, z site in the tile)
@@ -184,7 +189,7 @@ __global__ __launch_bounds__(256) void k_stage3(const F3Args<T> a) {
     // centre values and moment projections
     T c[kNVar3];
 #pragma unroll
-    for (int f = 0; f < kNVar3; ++f) c[f] = a.yin[o + (long long)f * g.nvp];
+    for (int f = 0; f < kNVar3; ++f) c[f] = yin[o + (long long)f * g.nvp];
     const double vxd = g.vx[v], vyd = g.vy[v], vzd = g.vz[v];
     cdouble3* m = (cdouble3*)(a.mom + ((long long)i * g.nz + j) * kNMom3);
     T P[8];
@@ -201,9 +206,15 @@ __global__ __launch_bounds__(256) void k_stage3(const F3Args<T> a) {
     const T fx = (T)(-vxd / g.twelve_dx), fz = (T)(-vzd / g.twelve_dz);
     const T kx = (T)g.ko_x, kz = (T)g.ko_z;
     const T cw = (T)a.cw, cy = (T)a.cy, ca = (T)a.ca, cb = (T)a.cb, third = (T)(1.0 / 3.0);
+    // FP32: all loads and arithmetic first, the stores after, so no load
+    // waits for a store (measured 5-10% faster); FP64: each field stored as
+    // soon as it is done (holding all 18 results costs a wave per SIMD, 4%
+    // slower)
+    constexpr bool kStoresLast = sizeof(T) == 4;
+    T yo[kStoresLast ? kNVar3 : 1], wn[kStoresLast && STAGE == 2 ? kNVar3 : 1];
 #pragma unroll
     for (int f = 0; f < kNVar3; ++f) {
-        const T* p = a.yin + o + (long long)f * g.nvp;
+        const T* p = yin + o + (long long)f * g.nvp;
         const T zm2 = p[zo[0]], zm1 = p[zo[1]], zp1 = p[zo[2]], zp2 = p[zo[3]];
         const T xm2 = p[-2 * xo], xm1 = p[-xo], xp1 = p[xo], xp2 = p[2 * xo];
         const T c0 = c[f];
@@ -211,22 +222,43 @@ __global__ __launch_bounds__(256) void k_stage3(const F3Args<T> a) {
         const T ko = kz * (zm2 + zp2 - (T)4 * (zm1 + zp1) + (T)6 * c0) + kx * (xm2 + xp2 - (T)4 * (xm1 + xp1) + (T)6 * c0);
         const T k = I[f] + adv + ko + V[f];
         const long long of = o + (long long)f * g.nvp;
+        T y, an = (T)0;
         if constexpr (STAGE == 0) {
-            a.yout[of] = k;
+            y = k;
         } else if constexpr (STAGE == 1 || STAGE == 5) {   // Y2 = u + dt/2 k1; SSP-RK3 u1 = u + dt k
-            a.yout[of] = c0 + cy * k;
+            y = c0 + cy * k;
         } else if constexpr (STAGE == 2) {                 // A = Y2 + u + dt k2, Y3 = u + dt/2 k2
             const T uc = a.u[of];
-            a.w[of] = (c0 + uc) + cw * k;
-            a.yout[of] = uc + cy * k;
+            an = (c0 + uc) + cw * k;
+            y = uc + cy * k;
         } else if constexpr (STAGE == 3) {                 // Y4 = u + dt k3
-            a.yout[of] = a.u[of] + cy * k;
+            y = a.u[of] + cy * k;
         } else if constexpr (STAGE == 4) {                 // u' = (A + Y4) / 3 + dt/6 k4
-            a.yout[of] = (a.w[of] + c0) * third + cw * k;
+            y = (a.w[of] + c0) * third + cw * k;
         } else {                                           // SSP-RK3 (nuosc_class.cpp:440, 444, 369)
-            a.yout[of] = ca * a.u[of] + cb * (c0 + cy * k);
+            y = ca * a.u[of] + cb * (c0 + cy * k);
+        }
+        if constexpr (kStoresLast) {
+            yo[f] = y;
+            if constexpr (STAGE == 2) wn[f] = an;
+        } else {
+            if constexpr (STAGE == 2) a.w[of] = an;
+            a.yout[of] = y;
         }
     }
+    if constexpr (kStoresLast) {
+#pragma unroll
+        for (int f = 0; f < kNVar3; ++f) {
+            const long long of = o + (long long)f * g.nvp;
+            if constexpr (STAGE == 2) a.w[of] = wn[f];
+            a.yout[of] = yo[f];
+        }
+    }
+}
+
+template <typename T, int STAGE>
+__global__ __launch_bounds__(256) void k_stage3(const F3Args<T> a) {
+    stage3_body<T, STAGE>(a, a.yin);
 }
 
 template <typename T, int STAGE>
