@@ -45,7 +45,7 @@ struct nuosc_sim {
     void* st_ya = nullptr;
     void* st_yb = nullptr;
     double* mom_part = nullptr;  // partial records of the last stage output
-    double* mom_u = nullptr;     // totals of u, yA, yB (3 flavours: mom_u alone, the stage input's moments)
+    double* mom_u = nullptr;     // totals of u, yA, yB (3 flavours: mom_u the stage input's, mom_a calRHS scratch)
     double* mom_a = nullptr;
     double* mom_b = nullptr;
     double* d_vgrid = nullptr;  // vx | vy | vz | vw
@@ -176,9 +176,11 @@ hipError_t stage_ghosts(nuosc_sim* s, T* y, double cdt) {
     return launch_xghosts<T>(s->geo, y, s->stream);
 }
 
-// 3 flavours: each stage is the ghost fill of its input, the moments of the
-// input, then the stage kernel (the RK4 combination of enqueue_step).  Events
-// ev[2k], ev[2k+1] bracket the stage-(k+1) kernel alone.
+// 3 flavours: each stage is the ghost fill of its input, the stage kernel
+// (the RK4 combination of enqueue_step; it reads the moment totals of its
+// input from mom_u and writes the partial moments of its output), then the
+// totals of the output into mom_u.  Events ev[2k], ev[2k+1] bracket the
+// stage-(k+1) kernel alone.
 template <typename T>
 int enqueue_step3(nuosc_sim* s, hipEvent_t* ev) {
     const double dt = s->dt, c6 = 1.0 / 6.0 * dt, c2 = 0.5 * dt;
@@ -188,6 +190,7 @@ int enqueue_step3(nuosc_sim* s, hipEvent_t* ev) {
     T* yb = static_cast<T*>(s->st_yb);
     const Geometry& g = s->geo;
     double* M = s->mom_u;
+    double* P = s->mom_part;
     auto mark = [&](int k) { return ev ? hipEventRecord(ev[k], s->stream) : hipSuccess; };
     struct St { int stage; T* yin; T* uu; T* ww; T* yout; double cw, cy, ct; };
     const St st[4] = {{1, u, nullptr, nullptr, ya, 0.0, c2, 0.0},
@@ -196,14 +199,17 @@ int enqueue_step3(nuosc_sim* s, hipEvent_t* ev) {
                       {4, ya, nullptr, w, u, c6, 0.0, dt}};
     for (int k = 0; k < 4; ++k) {
         HIPCK(s, stage_ghosts<T>(s, st[k].yin, st[k].ct));
-        HIPCK(s, launch_moments3<T>(g, st[k].yin, M, s->stream));
         HIPCK(s, mark(2 * k));
-        HIPCK(s, launch_stage3<T>(st[k].stage, g, s->ph, st[k].yin, st[k].uu, st[k].ww, st[k].yout, M, st[k].cw,
-                                  st[k].cy, s->stream));
+        HIPCK(s, launch_stage3<T>(st[k].stage, g, s->ph, st[k].yin, st[k].uu, st[k].ww, st[k].yout, M, P,
+                                  st[k].cw, st[k].cy, s->stream));
         HIPCK(s, mark(2 * k + 1));
+        HIPCK(s, launch_moment_totals3(g, P, M, s->stream));
     }
     if (g.bc == NUOSC_BC_OPEN_Z) HIPCK(s, launch_advance_time(s->d_time, dt, s->stream));
-    if (s->cfg.renorm) HIPCK(s, launch_renorm<T>(g, u, s->d_g0, s->d_g0 + s->nv, s->stream));
+    if (s->cfg.renorm) {
+        HIPCK(s, launch_renorm<T>(g, u, s->d_g0, s->d_g0 + s->nv, s->stream));
+        HIPCK(s, launch_moments3<T>(g, u, M, s->stream));
+    }
     return NUOSC_OK;
 }
 
@@ -216,17 +222,21 @@ int enqueue_step3_srk3(nuosc_sim* s) {
     T* yb = static_cast<T*>(s->st_yb);
     const Geometry& g = s->geo;
     double* M = s->mom_u;
+    double* P = s->mom_part;
     HIPCK(s, stage_ghosts<T>(s, u, 0.0));
-    HIPCK(s, launch_moments3<T>(g, u, M, s->stream));
-    HIPCK(s, launch_stage3<T>(5, g, s->ph, u, nullptr, nullptr, ya, M, 0.0, dt, s->stream));
+    HIPCK(s, launch_stage3<T>(5, g, s->ph, u, nullptr, nullptr, ya, M, P, 0.0, dt, s->stream));
+    HIPCK(s, launch_moment_totals3(g, P, M, s->stream));
     HIPCK(s, stage_ghosts<T>(s, ya, dt));
-    HIPCK(s, launch_moments3<T>(g, ya, M, s->stream));
-    HIPCK(s, launch_stage3<T>(6, g, s->ph, ya, u, nullptr, yb, M, 0.0, dt, s->stream, 3.0 / 4.0, 1.0 / 4.0));
+    HIPCK(s, launch_stage3<T>(6, g, s->ph, ya, u, nullptr, yb, M, P, 0.0, dt, s->stream, 3.0 / 4.0, 1.0 / 4.0));
+    HIPCK(s, launch_moment_totals3(g, P, M, s->stream));
     HIPCK(s, stage_ghosts<T>(s, yb, 0.5 * dt));
-    HIPCK(s, launch_moments3<T>(g, yb, M, s->stream));
-    HIPCK(s, launch_stage3<T>(7, g, s->ph, yb, u, nullptr, u, M, 0.0, dt, s->stream, 1.0 / 3.0, 2.0 / 3.0));
+    HIPCK(s, launch_stage3<T>(7, g, s->ph, yb, u, nullptr, u, M, P, 0.0, dt, s->stream, 1.0 / 3.0, 2.0 / 3.0));
+    HIPCK(s, launch_moment_totals3(g, P, M, s->stream));
     if (g.bc == NUOSC_BC_OPEN_Z) HIPCK(s, launch_advance_time(s->d_time, dt, s->stream));
-    if (s->cfg.renorm) HIPCK(s, launch_renorm<T>(g, u, s->d_g0, s->d_g0 + s->nv, s->stream));
+    if (s->cfg.renorm) {
+        HIPCK(s, launch_renorm<T>(g, u, s->d_g0, s->d_g0 + s->nv, s->stream));
+        HIPCK(s, launch_moments3<T>(g, u, M, s->stream));
+    }
     return NUOSC_OK;
 }
 
@@ -329,10 +339,11 @@ int enqueue_step_srk3(nuosc_sim* s) {
     return NUOSC_OK;
 }
 
-// partial records and totals of the state u (3 flavours: nothing to keep,
-// every stage computes the moments of its input)
+// partial records and totals of the state u (3 flavours: the totals)
 int refresh_moments(nuosc_sim* s) {
     if (s->nf == kNVar3) {
+        if (is64(s)) HIPCK(s, launch_moments3<double>(s->geo, (double*)s->st_u, s->mom_u, s->stream));
+        else HIPCK(s, launch_moments3<float>(s->geo, (float*)s->st_u, s->mom_u, s->stream));
         s->have_moments = true;
         return NUOSC_OK;
     }
@@ -529,7 +540,8 @@ int nuosc_create(const nuosc_config* cfg, nuosc_sim** out) {
 
     const size_t state_bytes = (size_t)s->state_elems * s->esize;
     const bool three = s->nf == kNVar3;
-    const size_t part_bytes = (size_t)(s->nx + 1) * s->nz * s->nch * kMomRec * sizeof(double);
+    const size_t part_bytes = three ? (size_t)s->nx * s->nz * ((s->nv + 63) / 64) * kNMom3 * sizeof(double)
+                                    : (size_t)(s->nx + 1) * s->nz * s->nch * kMomRec * sizeof(double);
     const size_t mom_bytes = (size_t)s->nx * s->nz * (three ? kNMom3 : kMomStride) * sizeof(double);
     void** states[4] = {&s->st_u, &s->st_w, &s->st_ya, &s->st_yb};
     {
@@ -542,7 +554,7 @@ int nuosc_create(const nuosc_config* cfg, nuosc_sim** out) {
     }
     double** moms[4] = {&s->mom_part, &s->mom_u, &s->mom_a, &s->mom_b};
     for (double** p : moms) {
-        if (three && p != &s->mom_u) continue;   // 3 flavours: one record of the stage input's moments
+        if (three && p == &s->mom_b) continue;   // 3 flavours: totals of the stage input and a calRHS scratch
         const size_t b = (p == &s->mom_part) ? part_bytes : mom_bytes;
         if (hipMalloc((void**)p, b) != hipSuccess) return bail(NUOSC_ENOMEM, "moment allocation failed");
         s->device_bytes += (long long)b;
@@ -980,13 +992,13 @@ int nuosc_calrhs(nuosc_sim* s, const double* in, double* out) {
     if (rc) return rc;
     if (s->nf == kNVar3) {
         if (is64(s)) {
-            HIPCK(s, launch_moments3<double>(s->geo, (double*)s->st_ya, s->mom_u, s->stream));
+            HIPCK(s, launch_moments3<double>(s->geo, (double*)s->st_ya, s->mom_a, s->stream));
             HIPCK(s, launch_stage3<double>(0, s->geo, s->ph, (double*)s->st_ya, nullptr, nullptr, (double*)s->st_yb,
-                                           s->mom_u, 0.0, 0.0, s->stream));
+                                           s->mom_a, nullptr, 0.0, 0.0, s->stream));
         } else {
-            HIPCK(s, launch_moments3<float>(s->geo, (float*)s->st_ya, s->mom_u, s->stream));
+            HIPCK(s, launch_moments3<float>(s->geo, (float*)s->st_ya, s->mom_a, s->stream));
             HIPCK(s, launch_stage3<float>(0, s->geo, s->ph, (float*)s->st_ya, nullptr, nullptr, (float*)s->st_yb,
-                                          s->mom_u, 0.0, 0.0, s->stream));
+                                          s->mom_a, nullptr, 0.0, 0.0, s->stream));
         }
     } else if (is64(s)) {
         HIPCK(s, launch_moments<double>(s->geo, (double*)s->st_ya, s->mom_part, s->stream));
@@ -1353,7 +1365,7 @@ int nuosc_estimate_device_bytes(const nuosc_config* c, long long* bytes) {
     const CopyLayout L = copy_layout((int)nx, (int)nz, (int)nzs, (int)nvp, (int)nch, esize, nf);
     if (!L.ok) return NUOSC_EINVAL;
     long long b = L.elems * esize;                                     // u, w, yA, yB
-    if (nf == kNVar3) b += nx * nz * kNMom3 * 8;                       // moments
+    if (nf == kNVar3) b += nx * nz * ((nv + 63) / 64 + 2) * kNMom3 * 8;   // moments: partials, 2 totals
     else b += (nx + 1) * nz * nch * kMomRec * 8 + 3ll * nx * nz * kMomStride * 8;
     b += 4 * nv * 8 + 2 * nv * 8 + 2048ll * kNAna * 16 + kNAna * 8;    // v-grid, G0, analysis
     b += 2 * nz * 8;                                                   // init tables

@@ -247,12 +247,15 @@ hipError_t launch_advance_time(double* t_dev, double dt, hipStream_t s);
 // (nuosc_class.cpp:28-82), summed over all directions of y in a fixed order.
 template <typename T>
 hipError_t launch_moments3(const Geometry& g, const T* y, double* mom, hipStream_t s);
-// One stage over the whole grid with the moments mom of yin; the stage
-// numbers and coefficients of launch_stage.
+// One stage over the whole grid with the moment totals mom of yin; the stage
+// numbers and coefficients of launch_stage.  Stages 1-7 also write the
+// partial moments of yout to part ([site][ceil(nv/64)][32] doubles), which
+// launch_moment_totals3 turns into the totals of yout.
 template <typename T>
 hipError_t launch_stage3(int stage, const Geometry& g, const Physics& ph, const T* yin, const T* u, T* w,
-                         T* yout, const double* mom, double cw, double cy, hipStream_t s, double ca = 0.0,
-                         double cb = 0.0);
+                         T* yout, const double* mom, double* part, double cw, double cy, hipStream_t s,
+                         double ca = 0.0, double cb = 0.0);
+hipError_t launch_moment_totals3(const Geometry& g, const double* part, double* mom, hipStream_t s);
 
 // Renormalisation of the state to |P| = 1 (the reference's formula).
 template <typename T>

@@ -2,17 +2,20 @@
 // (the reference's compile-time N_FLAVOR_3: 18 fields, nuosc_class.cpp:28-82,
 // 189-277; vacuum Hamiltonian :543-569).
 //
-// Every stage is two launches over the state copies (the layout of
-// nuosc_dev.h with nf = 18):
-//   k_moments3  one wavefront per site: the 32 velocity moments of the stage
-//               input, summed per lane over directions lane, lane+64, ... and
-//               then across the wave by halving exchanges (a fixed order, so a
-//               run's result does not depend on the launch);
+// Kernels over the state copies (the layout of nuosc_dev.h with nf = 18):
 //   k_stage3    one wavefront per (site, 64 directions), four x rows per
 //               block: the site's 18 centre values and 8 moment projections
 //               give the interaction and vacuum terms; each field's 8 stencil
 //               neighbours are then read straight from HBM/L2 and the stage
-//               combination of the RK4 or SSP-RK3 stage is stored.
+//               combination of the RK4 or SSP-RK3 stage is stored, and the
+//               32 moments of the stored values are summed over the wave's
+//               directions into a per-(site, chunk) partial record;
+//   k_moment_totals3  adds a site's partial records in chunk order: the
+//               moments the next stage reads (no extra pass over the state);
+//   k_moments3  one wavefront per site: the moments of a whole state (after
+//               init or upload, and for calRHS).
+// All sums run in a fixed order, so a run's result does not depend on the
+// launch.
 // The logical block order walks x quads inside z tiles of 16 sites, so the
 // z neighbours of a block were read a block or two earlier and its x
 // neighbours (the next quads) 16 blocks earlier, on the same XCD, and most
@@ -104,12 +107,20 @@ struct F3Args {
     const T* u;
     T* w;
     T* yout;
-    const double* mom;
+    const double* mom;   // moment totals of yin: [site][32]
+    double* part;        // partial moments of yout: [site][nch][32] (stages 1-7)
     double cw, cy, ca, cb;
     double mu, pmo;
     double h[9];
-    int tiles, quads;
+    int tiles, quads, nch;
 };
+
+// Moment differences (nuosc_class.cpp:57-71) a field's value enters, with
+// its sign: {d index, sign, second d index or -1, its sign}
+__device__ constexpr int kDiffOf[kNVar3][4] = {
+    {2, -1, -1, 0}, {2, +1, 7, -1}, {0, -1, -1, 0}, {1, +1, -1, 0}, {2, +1, -1, 0}, {2, -1, 7, +1},
+    {0, +1, -1, 0}, {1, +1, -1, 0}, {7, +1, -1, 0}, {3, -1, -1, 0}, {4, +1, -1, 0}, {5, -1, -1, 0},
+    {6, +1, -1, 0}, {7, -1, -1, 0}, {3, +1, -1, 0}, {4, +1, -1, 0}, {5, +1, -1, 0}, {6, +1, -1, 0}};
 
 // Interaction (nuosc_class.cpp:206-252) and vacuum (:260-277) terms of one
 // flavour block (nu: o = 0, nubar: o = 1) from the centre values c and the
@@ -173,8 +184,12 @@ __device__ __forceinline__ void stage3_body(const F3Args<T>& a, const T* __restr
     const int ch = r / a.tiles;
     const int j = jt * kF3Tile + jj;
     const int i = __builtin_amdgcn_readfirstlane(qx * kF3Rows + (int)(threadIdx.x >> 6));
-    const int v = ch * 64 + (int)(threadIdx.x & 63);
-    if (j >= g.nz || i >= g.nx || v >= g.nv) return;
+    const int lane = (int)(threadIdx.x & 63);
+    if (j >= g.nz || i >= g.nx) return;   // whole wavefronts
+    // lanes past the last direction compute on the last one, store nothing
+    // and add nothing to the moments (all lanes take part in the reduction)
+    const bool live = ch * 64 + lane < g.nv;
+    const int v = live ? ch * 64 + lane : g.nv - 1;
 
     const long long o = elem_off(g, i, j + g.zoff, 0) + v;
     long long zo[4];   // z - 2, z - 1, z + 1, z + 2: stored ghost rows (open z) or the periodic wrap
@@ -212,6 +227,7 @@ __device__ __forceinline__ void stage3_body(const F3Args<T>& a, const T* __restr
     // slower)
     constexpr bool kStoresLast = sizeof(T) == 4;
     T yo[kStoresLast ? kNVar3 : 1], wn[kStoresLast && STAGE == 2 ? kNVar3 : 1];
+    double d[8] = {0, 0, 0, 0, 0, 0, 0, 0};   // moment differences of the output (stages 1-7)
 #pragma unroll
     for (int f = 0; f < kNVar3; ++f) {
         const T* p = yin + o + (long long)f * g.nvp;
@@ -238,27 +254,61 @@ __device__ __forceinline__ void stage3_body(const F3Args<T>& a, const T* __restr
         } else {                                           // SSP-RK3 (nuosc_class.cpp:440, 444, 369)
             y = ca * a.u[of] + cb * (c0 + cy * k);
         }
+        if constexpr (STAGE != 0) {
+            d[kDiffOf[f][0]] += kDiffOf[f][1] > 0 ? (double)y : -(double)y;
+            if (kDiffOf[f][2] >= 0) d[kDiffOf[f][2]] += kDiffOf[f][3] > 0 ? (double)y : -(double)y;
+        }
         if constexpr (kStoresLast) {
             yo[f] = y;
             if constexpr (STAGE == 2) wn[f] = an;
-        } else {
+        } else if (live) {
             if constexpr (STAGE == 2) a.w[of] = an;
             a.yout[of] = y;
         }
     }
     if constexpr (kStoresLast) {
+        if (live) {
 #pragma unroll
-        for (int f = 0; f < kNVar3; ++f) {
-            const long long of = o + (long long)f * g.nvp;
-            if constexpr (STAGE == 2) a.w[of] = wn[f];
-            a.yout[of] = yo[f];
+            for (int f = 0; f < kNVar3; ++f) {
+                const long long of = o + (long long)f * g.nvp;
+                if constexpr (STAGE == 2) a.w[of] = wn[f];
+                a.yout[of] = yo[f];
+            }
         }
+    }
+    if constexpr (STAGE != 0) {
+        // partial moments of the output over this wave's directions
+        // (nuosc_class.cpp:65-71), reduced across the wave in a fixed order
+        const double wv = live ? g.vw[v] : 0.0;
+        const double q4[4] = {wv, vxd * wv, vyd * wv, vzd * wv};
+        double t[32];
+#pragma unroll
+        for (int cc = 0; cc < 8; ++cc)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) t[4 * cc + q] = q4[q] * d[cc];
+        const double tot = wave_sum32(t, lane);
+        if ((lane & 1) == 0)
+            a.part[(((long long)i * g.nz + j) * a.nch + ch) * kNMom3 + (lane >> 1)] = tot;
     }
 }
 
+// 4 waves per SIMD (128 VGPRs; FP64 stages 2-4, 6, 7 spill 2 registers):
+// measured 7% faster per FP64 step than the 3 waves the compiler picks alone
 template <typename T, int STAGE>
-__global__ __launch_bounds__(256) void k_stage3(const F3Args<T> a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_stage3(const F3Args<T> a) {
     stage3_body<T, STAGE>(a, a.yin);
+}
+
+// per-site totals of the partial records, summed in chunk order
+__global__ __launch_bounds__(256) void k_moment_totals3(const double* __restrict__ part, int nch, long long n,
+                                                        double* __restrict__ mom) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;   // site * 32 + value
+    if (k >= n) return;
+    const long long site = k / kNMom3, q = k % kNMom3;
+    const double* p = part + site * nch * kNMom3 + q;
+    double s = p[0];
+    for (int c = 1; c < nch; ++c) s += p[(long long)c * kNMom3];
+    mom[k] = s;
 }
 
 template <typename T, int STAGE>
@@ -277,10 +327,19 @@ hipError_t launch_moments3(const Geometry& g, const T* y, double* mom, hipStream
     return hipGetLastError();
 }
 
+hipError_t launch_moment_totals3(const Geometry& g, const double* part, double* mom, hipStream_t s) {
+    const long long n = (long long)g.nx * g.nz * kNMom3;
+    if (g.nf != kNVar3 || n <= 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_moment_totals3, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, part,
+                       (g.nv + 63) / 64, n, mom);
+    return hipGetLastError();
+}
+
 template <typename T>
 hipError_t launch_stage3(int stage, const Geometry& g, const Physics& ph, const T* yin, const T* u, T* w, T* yout,
-                         const double* mom, double cw, double cy, hipStream_t s, double ca, double cb) {
-    if (g.nf != kNVar3) return hipErrorInvalidValue;
+                         const double* mom, double* part, double cw, double cy, hipStream_t s, double ca,
+                         double cb) {
+    if (g.nf != kNVar3 || (stage != 0 && !part)) return hipErrorInvalidValue;
     F3Args<T> a{};
     a.g = g;
     a.yin = yin;
@@ -288,6 +347,8 @@ hipError_t launch_stage3(int stage, const Geometry& g, const Physics& ph, const 
     a.w = w;
     a.yout = yout;
     a.mom = mom;
+    a.part = part;
+    a.nch = (g.nv + 63) / 64;
     a.cw = cw;
     a.cy = cy;
     a.ca = ca;
@@ -297,8 +358,7 @@ hipError_t launch_stage3(int stage, const Geometry& g, const Physics& ph, const 
     for (int q = 0; q < 9; ++q) a.h[q] = ph.h3[q];
     a.tiles = (g.nz + kF3Tile - 1) / kF3Tile;
     a.quads = (g.nx + kF3Rows - 1) / kF3Rows;
-    const long long nch = (g.nv + 63) / 64;
-    const long long nb = nch * a.tiles * a.quads * kF3Tile;
+    const long long nb = (long long)a.nch * a.tiles * a.quads * kF3Tile;
     if (nb <= 0 || nb > 0x7fffffffLL) return hipErrorInvalidValue;
     switch (stage) {
         case 0: return launch3<T, 0>(a, (unsigned)nb, s);
@@ -316,9 +376,10 @@ hipError_t launch_stage3(int stage, const Geometry& g, const Physics& ph, const 
 template hipError_t launch_moments3<double>(const Geometry&, const double*, double*, hipStream_t);
 template hipError_t launch_moments3<float>(const Geometry&, const float*, double*, hipStream_t);
 template hipError_t launch_stage3<double>(int, const Geometry&, const Physics&, const double*, const double*,
-                                          double*, double*, const double*, double, double, hipStream_t, double,
-                                          double);
+                                          double*, double*, const double*, double*, double, double, hipStream_t,
+                                          double, double);
 template hipError_t launch_stage3<float>(int, const Geometry&, const Physics&, const float*, const float*, float*,
-                                         float*, const double*, double, double, hipStream_t, double, double);
+                                         float*, const double*, double*, double, double, hipStream_t, double,
+                                         double);
 
 }  // namespace nuosc
