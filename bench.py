@@ -21,10 +21,9 @@ Prints one JSON line:
   cpu_baseline   the CPU oracle on a 128 x 128 x 642 crop of the same physics,
                  timed on this host's cores (a reported baseline)
 
---nflavor 3 runs the same box with 3 flavours (18 fields, pmo=1): every stage
-is a moment pass over its input plus the stage kernel, so the stage passes
-are (3, 5, 4, 4) of 18 fields; the roofline line is the stage-2 kernel alone
-(4 passes).
+--nflavor 3 runs the same box with 3 flavours (18 fields, pmo=1): the same
+12 passes per step, of 18 fields; the roofline line is the stage-2 kernel
+(4 passes of 18 fields).
 """
 from __future__ import annotations
 
@@ -361,8 +360,7 @@ def main():
     esize = 8 if a.precision == 64 else 4
     three = a.nflavor == 3
     nvar = 18 if three else 8
-    # 3 flavours: the moment pass of each stage reads its input once more
-    passes = tuple(p + 1 for p in STAGE_PASSES) if three else STAGE_PASSES
+    passes = STAGE_PASSES
     pmo = 1.0 if three else 0.0
     cfg = dict(nv=a.nn, x0=-a.zmax, x1=a.zmax, z0=-a.zmax, z1=a.zmax, dx=a.dz, dz=a.dz, CFL=0.4, ko=0.1,
                mu=1.0, pmo=pmo, precision=prec, use_graph=True, nflavor=a.nflavor)
