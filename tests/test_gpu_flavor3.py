@@ -9,6 +9,8 @@ nuosc_class.cpp:28-82, 189-277, 543-569) against the oracle:
   * an open-z column, 20 RK4 steps: relative L-inf <= 1e-10 per field;
   * calRHS of the FP32 form against FP64 on a state exactly representable in
     FP32: within 64 eps32 max|field| / h (FP32 rounding of the stencil);
+  * renormalising runs, 10 steps: relative L-inf <= 1e-10 per field;
+  * hipGraph replay bitwise equal to eager launches; per-stage timing;
   * the CLI with --nflavor 3, and the arguments the path refuses.
 """
 import os
@@ -188,3 +190,47 @@ def test_cli_nflavor3(tmp_path):
     assert "Fieldvar: 18" in p.stdout
     rows = [l for l in open(tmp_path / "analysis.dat").read().splitlines()[1:] if l.strip()]
     assert len(rows) == 3
+
+
+def test_flavor3_graph_replay_bitwise(nuosc_mod):
+    """hipGraph replay of the 3-flavour step (stage kernels + moment totals)
+    equals eager launches bit for bit, over several replays."""
+    N = nuosc_mod
+    c = cfg_for(9, 10, 2, ko=0.05, pmo=0.5)
+    init = dict(ipt=5, alpha=0.9, eps0=0.2, sigma=1.0, lnue=0.6, lnueb=0.5, seed=21)
+    a = N.NuOsc(**c, nflavor=3, use_graph=True)
+    b = N.NuOsc(**c, nflavor=3, use_graph=False)
+    for g in (a, b):
+        g.fillInitValue(**init)
+    for _ in range(3):
+        a.step_rk4(4)
+        b.step_rk4(4)
+        assert np.array_equal(a.get_state(), b.get_state())
+    a.close()
+    b.close()
+
+
+def test_flavor3_renorm(oracle_mod, nuosc_mod):
+    O, N = oracle_mod, nuosc_mod
+    c = cfg_for(6, 8, 2, ko=0.05, pmo=0.5)
+    o = oracle_sim(O, c, renorm=1)
+    g = N.NuOsc(**c, nflavor=3, renorm=True)
+    init = dict(ipt=5, alpha=0.9, eps0=0.2, sigma=1.0, lnue=0.6, lnueb=0.5, seed=4)
+    o.fill_init(**init)
+    g.fillInitValue(**init)
+    o.step(10)
+    g.step_rk4(10)
+    gs, os_ = g.get_state(), o.get_state()
+    for f in range(18):
+        assert rel_linf(gs[f], os_[f]) <= 1e-10, (f, rel_linf(gs[f], os_[f]))
+    g.close()
+    o.close()
+
+
+def test_flavor3_profile_stages(nuosc_mod):
+    N = nuosc_mod
+    g = N.NuOsc(**cfg_for(8, 16, 2), nflavor=3)
+    g.fillInitValue(ipt=5, alpha=0.9, eps0=0.1, sigma=1.0, lnue=0.6, lnueb=0.5, seed=1)
+    ms = g.profile_stages(2)
+    assert len(ms) == 4 and all(x > 0 for x in ms)
+    g.close()
