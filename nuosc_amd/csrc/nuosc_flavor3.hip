@@ -168,11 +168,37 @@ __device__ __forceinline__ void local_terms3(const T (&c)[kNVar3], const T (&P)[
     V[BTT] = V[TT];
 }
 
+// Orders the LDS accesses of the block (LDS-only fences around the
+// barrier, so outstanding global loads are not waited for).
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Fields staged in LDS at a time: the x rows of the block plus 2 edge rows
+// on each side, 64 directions: 36.9 KB per block in both precisions.
+template <typename T>
+constexpr int kGrp3 = sizeof(T) == 4 ? 18 : 9;
+
+template <typename T>
+struct XSlab {
+    T x[kGrp3<T>][kF3Rows + 4][64];
+};
+
 // The stage input is never one of the stage's outputs (yin differs from u, w
 // and yout in every stage): declared restrict, the stencil loads of later
 // fields can be issued before the stores of earlier ones.
+//
+// x neighbours come from LDS: every wave writes its own row's centre values
+// and one of the 4 rows beyond the block's edges (wave 0: i0-2, 1: i0-1,
+// 2: i0+4, 3: i0+5), so each field costs 1 global load for the x stencil
+// instead of 4.  All waves of the block reach every barrier; waves whose row
+// is past nx load a stored row (at most nx + 1, a periodic ghost row) and
+// compute nothing.
 template <typename T, int STAGE>
-__device__ __forceinline__ void stage3_body(const F3Args<T>& a, const T* __restrict__ yin) {
+__device__ __forceinline__ void stage3_body(const F3Args<T>& a, const T* __restrict__ yin, XSlab<T>& sx) {
+    constexpr int kGrp = kGrp3<T>;
     const Geometry& g = a.g;
     const int lb = xcd_remap(blockIdx.x, gridDim.x);
     // logical block -> (direction chunk, z tile, x quad, z site in the tile)
@@ -183,15 +209,22 @@ __device__ __forceinline__ void stage3_body(const F3Args<T>& a, const T* __restr
     const int jt = r % a.tiles;
     const int ch = r / a.tiles;
     const int j = jt * kF3Tile + jj;
-    const int i = __builtin_amdgcn_readfirstlane(qx * kF3Rows + (int)(threadIdx.x >> 6));
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int i = qx * kF3Rows + w;
     const int lane = (int)(threadIdx.x & 63);
-    if (j >= g.nz || i >= g.nx) return;   // whole wavefronts
+    if (j >= g.nz) return;   // whole blocks
+    const bool row_ok = i < g.nx;
+    const int il = min(i, g.nx + 1);                                  // row loaded
+    const int er = w < 2 ? qx * kF3Rows - 2 + w : qx * kF3Rows + 2 + w;   // edge row of this wave
+    const int el = min(er, g.nx + 1);
+    const int eslot = w < 2 ? w : w + 4;
     // lanes past the last direction compute on the last one, store nothing
     // and add nothing to the moments (all lanes take part in the reduction)
     const bool live = ch * 64 + lane < g.nv;
     const int v = live ? ch * 64 + lane : g.nv - 1;
 
-    const long long o = elem_off(g, i, j + g.zoff, 0) + v;
+    const long long o = elem_off(g, il, j + g.zoff, 0) + v;
+    const long long oe = elem_off(g, el, j + g.zoff, 0) + v;
     long long zo[4];   // z - 2, z - 1, z + 1, z + 2: stored ghost rows (open z) or the periodic wrap
     {
         const int ks[4] = {-2, -1, 1, 2};
@@ -199,14 +232,13 @@ __device__ __forceinline__ void stage3_body(const F3Args<T>& a, const T* __restr
         for (int q = 0; q < 4; ++q)
             zo[q] = (long long)(g.bc ? ks[q] : wrapi(j + ks[q], g.nz) - j) * g.srec;
     }
-    const long long xo = g.xrow;
 
     // centre values and moment projections
     T c[kNVar3];
 #pragma unroll
     for (int f = 0; f < kNVar3; ++f) c[f] = yin[o + (long long)f * g.nvp];
     const double vxd = g.vx[v], vyd = g.vy[v], vzd = g.vz[v];
-    cdouble3* m = (cdouble3*)(a.mom + ((long long)i * g.nz + j) * kNMom3);
+    cdouble3* m = (cdouble3*)(a.mom + ((long long)min(i, g.nx - 1) * g.nz + j) * kNMom3);
     T P[8];
 #pragma unroll
     for (int q = 0; q < 8; ++q)   // INP, nuosc_class.cpp:187
@@ -223,51 +255,68 @@ __device__ __forceinline__ void stage3_body(const F3Args<T>& a, const T* __restr
     const T cw = (T)a.cw, cy = (T)a.cy, ca = (T)a.ca, cb = (T)a.cb, third = (T)(1.0 / 3.0);
     // FP32: all loads and arithmetic first, the stores after, so no load
     // waits for a store (measured 5-10% faster); FP64: each field stored as
-    // soon as it is done (holding all 18 results costs a wave per SIMD, 4%
-    // slower)
+    // soon as it is done (holding all 18 results costs a wave per SIMD)
     constexpr bool kStoresLast = sizeof(T) == 4;
     T yo[kStoresLast ? kNVar3 : 1], wn[kStoresLast && STAGE == 2 ? kNVar3 : 1];
     double d[8] = {0, 0, 0, 0, 0, 0, 0, 0};   // moment differences of the output (stages 1-7)
+    const bool put = row_ok && live;
 #pragma unroll
-    for (int f = 0; f < kNVar3; ++f) {
-        const T* p = yin + o + (long long)f * g.nvp;
-        const T zm2 = p[zo[0]], zm1 = p[zo[1]], zp1 = p[zo[2]], zp2 = p[zo[3]];
-        const T xm2 = p[-2 * xo], xm1 = p[-xo], xp1 = p[xo], xp2 = p[2 * xo];
-        const T c0 = c[f];
-        const T adv = fz * ((zm2 - zp2) - (T)8.0 * (zm1 - zp1)) + fx * ((xm2 - xp2) - (T)8.0 * (xm1 - xp1));
-        const T ko = kz * (zm2 + zp2 - (T)4 * (zm1 + zp1) + (T)6 * c0) + kx * (xm2 + xp2 - (T)4 * (xm1 + xp1) + (T)6 * c0);
-        const T k = I[f] + adv + ko + V[f];
-        const long long of = o + (long long)f * g.nvp;
-        T y, an = (T)0;
-        if constexpr (STAGE == 0) {
-            y = k;
-        } else if constexpr (STAGE == 1 || STAGE == 5) {   // Y2 = u + dt/2 k1; SSP-RK3 u1 = u + dt k
-            y = c0 + cy * k;
-        } else if constexpr (STAGE == 2) {                 // A = Y2 + u + dt k2, Y3 = u + dt/2 k2
-            const T uc = a.u[of];
-            an = (c0 + uc) + cw * k;
-            y = uc + cy * k;
-        } else if constexpr (STAGE == 3) {                 // Y4 = u + dt k3
-            y = a.u[of] + cy * k;
-        } else if constexpr (STAGE == 4) {                 // u' = (A + Y4) / 3 + dt/6 k4
-            y = (a.w[of] + c0) * third + cw * k;
-        } else {                                           // SSP-RK3 (nuosc_class.cpp:440, 444, 369)
-            y = ca * a.u[of] + cb * (c0 + cy * k);
+    for (int g0 = 0; g0 < kNVar3; g0 += kGrp) {
+        if (g0 > 0) lds_barrier();   // every wave has read the previous group
+#pragma unroll
+        for (int q = 0; q < kGrp; ++q) {
+            sx.x[q][w + 2][lane] = c[g0 + q];
+            sx.x[q][eslot][lane] = yin[oe + (long long)(g0 + q) * g.nvp];
         }
-        if constexpr (STAGE != 0) {
-            d[kDiffOf[f][0]] += kDiffOf[f][1] > 0 ? (double)y : -(double)y;
-            if (kDiffOf[f][2] >= 0) d[kDiffOf[f][2]] += kDiffOf[f][3] > 0 ? (double)y : -(double)y;
-        }
-        if constexpr (kStoresLast) {
-            yo[f] = y;
-            if constexpr (STAGE == 2) wn[f] = an;
-        } else if (live) {
-            if constexpr (STAGE == 2) a.w[of] = an;
-            a.yout[of] = y;
+        lds_barrier();
+        if (row_ok) {
+#pragma unroll
+            for (int q = 0; q < kGrp; ++q) {
+                const int f = g0 + q;
+                const T* p = yin + o + (long long)f * g.nvp;
+                const T zm2 = p[zo[0]], zm1 = p[zo[1]], zp1 = p[zo[2]], zp2 = p[zo[3]];
+                const T xm2 = sx.x[q][w][lane], xm1 = sx.x[q][w + 1][lane];
+                const T xp1 = sx.x[q][w + 3][lane], xp2 = sx.x[q][w + 4][lane];
+                const T c0 = c[f];
+                const T adv =
+                    fz * ((zm2 - zp2) - (T)8.0 * (zm1 - zp1)) + fx * ((xm2 - xp2) - (T)8.0 * (xm1 - xp1));
+                const T ko = kz * (zm2 + zp2 - (T)4 * (zm1 + zp1) + (T)6 * c0) +
+                             kx * (xm2 + xp2 - (T)4 * (xm1 + xp1) + (T)6 * c0);
+                const T k = I[f] + adv + ko + V[f];
+                const long long of = o + (long long)f * g.nvp;
+                T y, an = (T)0;
+                if constexpr (STAGE == 0) {
+                    y = k;
+                } else if constexpr (STAGE == 1 || STAGE == 5) {   // Y2 = u + dt/2 k1; SSP-RK3 u1 = u + dt k
+                    y = c0 + cy * k;
+                } else if constexpr (STAGE == 2) {                 // A = Y2 + u + dt k2, Y3 = u + dt/2 k2
+                    const T uc = a.u[of];
+                    an = (c0 + uc) + cw * k;
+                    y = uc + cy * k;
+                } else if constexpr (STAGE == 3) {                 // Y4 = u + dt k3
+                    y = a.u[of] + cy * k;
+                } else if constexpr (STAGE == 4) {                 // u' = (A + Y4) / 3 + dt/6 k4
+                    y = (a.w[of] + c0) * third + cw * k;
+                } else {                                           // SSP-RK3 (nuosc_class.cpp:440, 444, 369)
+                    y = ca * a.u[of] + cb * (c0 + cy * k);
+                }
+                if constexpr (STAGE != 0) {
+                    d[kDiffOf[f][0]] += kDiffOf[f][1] > 0 ? (double)y : -(double)y;
+                    if (kDiffOf[f][2] >= 0) d[kDiffOf[f][2]] += kDiffOf[f][3] > 0 ? (double)y : -(double)y;
+                }
+                if constexpr (kStoresLast) {
+                    yo[f] = y;
+                    if constexpr (STAGE == 2) wn[f] = an;
+                } else if (live) {
+                    if constexpr (STAGE == 2) a.w[of] = an;
+                    a.yout[of] = y;
+                }
+            }
         }
     }
+    if (!row_ok) return;   // after the last barrier
     if constexpr (kStoresLast) {
-        if (live) {
+        if (put) {
 #pragma unroll
             for (int f = 0; f < kNVar3; ++f) {
                 const long long of = o + (long long)f * g.nvp;
@@ -296,7 +345,8 @@ __device__ __forceinline__ void stage3_body(const F3Args<T>& a, const T* __restr
 // measured 7% faster per FP64 step than the 3 waves the compiler picks alone
 template <typename T, int STAGE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_stage3(const F3Args<T> a) {
-    stage3_body<T, STAGE>(a, a.yin);
+    __shared__ XSlab<T> sx;
+    stage3_body<T, STAGE>(a, a.yin, sx);
 }
 
 // per-site totals of the partial records, summed in chunk order
