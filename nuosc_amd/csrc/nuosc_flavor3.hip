@@ -5,9 +5,11 @@
 // Kernels over the state copies (the layout of nuosc_dev.h with nf = 18):
 //   k_stage3    one wavefront per (site, 64 directions), four x rows per
 //               block: the site's 18 centre values and 8 moment projections
-//               give the interaction and vacuum terms; each field's 8 stencil
-//               neighbours are then read straight from HBM/L2 and the stage
-//               combination of the RK4 or SSP-RK3 stage is stored, and the
+//               give the interaction and vacuum terms; each field's x
+//               neighbours come from the block's LDS (the waves' own rows and
+//               one edge row loaded by each wave), its z neighbours straight
+//               from HBM/L2; the combination of the RK4 or SSP-RK3 stage is
+//               stored, and the
 //               32 moments of the stored values are summed over the wave's
 //               directions into a per-(site, chunk) partial record;
 //   k_moment_totals3  adds a site's partial records in chunk order: the
@@ -20,9 +22,9 @@
 // z neighbours of a block were read a block or two earlier and its x
 // neighbours (the next quads) 16 blocks earlier, on the same XCD, and most
 // stencil re-reads hit L2 (HBM reads 1.3-1.6x one pass of the stage input at
-// 512 x 512 x 642).  The cost is the 8 neighbour loads per field and lane:
-// with 18 fields a register z-window like the 2-flavour kernel's would take
-// 180 VGPRs in FP64.
+// 512 x 512 x 642).  The cost is the number of neighbour loads per field
+// and lane: with 18 fields a register z-window like the 2-flavour kernel's
+// would take 180 VGPRs in FP64.
 #include <hip/hip_runtime.h>
 
 #include "nuosc_dev.h"
