@@ -287,7 +287,9 @@ int nuosc_estimate_device_bytes(const nuosc_config* cfg, long long* bytes);
  * while their summed device footprint fits the HBM budget, advances all
  * resident runs together through a batch (continuous batching: runs join and
  * leave between ticks of at most max_tick_steps steps) and records each run's
- * (t, Pee, Peebar, maxdP, |M0|) series every ana_every steps.
+ * (t, Pee, Peebar, maxdP, |M0|) series every ana_every steps.  submit
+ * refuses (NUOSC_EINVAL) runs a batch cannot take: open z, a non-default
+ * discretisation, 3 flavours.
  * ------------------------------------------------------------------------ */
 typedef struct nuosc_server nuosc_server;
 

@@ -458,9 +458,9 @@ class Server:
 
     def submit(self, nsteps, ana_every=10, ipt=5, alpha=0.9, eps0=1e-6, sigma=1.0, lnue=0.6, lnueb=0.5,
                lnuex=DBL_MAX, lnuebx=DBL_MAX, seed=0, nv=4, x0=-1.0, x1=1.0, z0=-1.0, z1=1.0, dx=0.1, dz=0.1,
-               CFL=0.4, ko=0.1, mu=1.0, pmo=0.0):
+               CFL=0.4, ko=0.1, mu=1.0, pmo=0.0, nflavor=2):
         cfg = Config(nn=nv, gx=2, gz=2, x0=x0, x1=x1, z0=z0, z1=z1, dx=dx, dz=dz, cfl=CFL, ko=ko, mu=mu, pmo=pmo,
-                     renorm=0, vgrid=0, precision=0, bc=0, device=0, use_graph=0)
+                     renorm=0, vgrid=0, precision=0, bc=0, device=0, use_graph=0, nflavor=nflavor)
         req = Request(cfg=cfg, ipt=ipt, alpha=alpha, eps0=eps0, sigma=sigma, lnue=lnue, lnueb=lnueb, lnuex=lnuex,
                       lnuebx=lnuebx, seed=seed, nsteps=nsteps, ana_every=ana_every)
         rid = C.c_longlong()

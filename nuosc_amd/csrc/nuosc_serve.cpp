@@ -219,6 +219,9 @@ int nuosc_server_submit(nuosc_server* s, const nuosc_request* req, long long* id
     long long bytes = 0;
     nuosc_config c = req->cfg;
     c.precision = s->precision;
+    // runs the batch cannot take (nuosc_batch_add) are refused here, not at admission
+    if (c.bc != NUOSC_BC_PERIODIC || c.ko_order == 5 || c.adv != NUOSC_ADV_CFD4 || c.nflavor == 3)
+        return NUOSC_EINVAL;
     if (nuosc_estimate_device_bytes(&c, &bytes) != NUOSC_OK) return NUOSC_EINVAL;
     Run* r = new Run();
     r->req = *req;

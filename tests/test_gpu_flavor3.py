@@ -234,3 +234,11 @@ def test_flavor3_profile_stages(nuosc_mod):
     ms = g.profile_stages(2)
     assert len(ms) == 4 and all(x > 0 for x in ms)
     g.close()
+
+
+def test_server_refuses_flavor3(nuosc_mod):
+    N = nuosc_mod
+    srv = N.Server(precision=N.PREC_FP64, hbm_budget_bytes=1 << 30)
+    with pytest.raises(N.NuOscError):
+        srv.submit(10, nflavor=3)
+    srv.close()
