@@ -185,7 +185,8 @@ constexpr int kGrp3 = sizeof(T) == 4 ? 18 : 9;
 
 template <typename T>
 struct XSlab {
-    T x[kGrp3<T>][kF3Rows + 4][64];
+    // rows i0-2 .. i0+R+1, then R-4 slots that waves 4.. write unread
+    T x[kGrp3<T>][2 * kF3Rows][64];
 };
 
 // The stage input is never one of the stage's outputs (yin differs from u, w
@@ -217,9 +218,13 @@ __device__ __forceinline__ void stage3_body(const F3Args<T>& a, const T* __restr
     if (j >= g.nz) return;   // whole blocks
     const bool row_ok = i < g.nx;
     const int il = min(i, g.nx + 1);                                  // row loaded
-    const int er = w < 2 ? qx * kF3Rows - 2 + w : qx * kF3Rows + 2 + w;   // edge row of this wave
+    // edge rows i0-2, i0-1, i0+R, i0+R+1 (LDS slots 0, 1, R+2, R+3), one for
+    // each of waves 0-3; waves 4.. reload their own row into a slot nothing
+    // reads (unconditional, so the compiler keeps the loads in flight)
+    static_assert(kF3Rows >= 4, "four waves load the edge rows");
+    const int er = w < 2 ? qx * kF3Rows - 2 + w : (w < 4 ? qx * kF3Rows + kF3Rows - 2 + w : il);
     const int el = min(er, g.nx + 1);
-    const int eslot = w < 2 ? w : w + 4;
+    const int eslot = w < 2 ? w : w + kF3Rows;
     // lanes past the last direction compute on the last one, store nothing
     // and add nothing to the moments (all lanes take part in the reduction)
     const bool live = ch * 64 + lane < g.nv;
@@ -346,7 +351,7 @@ __device__ __forceinline__ void stage3_body(const F3Args<T>& a, const T* __restr
 // 4 waves per SIMD (128 VGPRs; FP64 stages 2-4, 6, 7 spill 2 registers):
 // measured 7% faster per FP64 step than the 3 waves the compiler picks alone
 template <typename T, int STAGE>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_stage3(const F3Args<T> a) {
+__global__ __launch_bounds__(64 * kF3Rows) __attribute__((amdgpu_waves_per_eu(4))) void k_stage3(const F3Args<T> a) {
     __shared__ XSlab<T> sx;
     stage3_body<T, STAGE>(a, a.yin, sx);
 }
