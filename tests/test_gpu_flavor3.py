@@ -9,6 +9,8 @@ nuosc_class.cpp:28-82, 189-277, 543-569) against the oracle:
   * an open-z column, 20 RK4 steps: relative L-inf <= 1e-10 per field;
   * calRHS of the FP32 form against FP64 on a state exactly representable in
     FP32: within 64 eps32 max|field| / h (FP32 rounding of the stencil);
+  * 50 RK4 steps of the FP32 form against FP64: Pee within 1e-6, fields
+    within 1e-4 of max|field|;
   * renormalising runs, 10 steps: relative L-inf <= 1e-10 per field;
   * hipGraph replay bitwise equal to eager launches; per-stage timing;
   * the CLI with --nflavor 3, and the arguments the path refuses.
@@ -242,3 +244,23 @@ def test_server_refuses_flavor3(nuosc_mod):
     with pytest.raises(N.NuOscError):
         srv.submit(10, nflavor=3)
     srv.close()
+
+
+def test_flavor3_fp32_tracks_fp64(nuosc_mod):
+    """50 RK4 steps of the FP32 form against FP64 from the same seeded
+    state: Pee within 1e-6, fields within 1e-4 of max|field| (FP32 rounding
+    accumulated over the run)."""
+    N = nuosc_mod
+    c = cfg_for(12, 16, 4, ko=0.05, pmo=0.5)
+    init = dict(ipt=5, alpha=0.9, eps0=1e-3, sigma=1.0, lnue=0.6, lnueb=0.5, seed=8)
+    a = N.NuOsc(**c, nflavor=3)
+    b = N.NuOsc(**c, nflavor=3, precision=N.PREC_FP32)
+    for g in (a, b):
+        g.fillInitValue(**init)
+        g.step_rk4(50)
+    sa, sb = a.get_state(), b.get_state()
+    assert np.max(np.abs(sa - sb)) <= 1e-4 * np.max(np.abs(sa))
+    assert np.any(sa[8:])                        # the vacuum term populated the tau fields
+    assert abs(a.analysis().surv - b.analysis().surv) <= 1e-6
+    a.close()
+    b.close()
