@@ -9,9 +9,9 @@
 //               neighbours come from the block's LDS (the waves' own rows and
 //               one edge row loaded by each wave), its z neighbours straight
 //               from HBM/L2; the combination of the RK4 or SSP-RK3 stage is
-//               stored, and the
-//               32 moments of the stored values are summed over the wave's
-//               directions into a per-(site, chunk) partial record;
+//               stored, and the 32 moments of the stored values are summed
+//               over the wave's directions into a per-(site, chunk) partial
+//               record;
 //   k_moment_totals3  adds a site's partial records in chunk order: the
 //               moments the next stage reads (no extra pass over the state);
 //   k_moments3  one wavefront per site: the moments of a whole state (after
