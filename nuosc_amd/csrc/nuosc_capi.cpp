@@ -265,7 +265,8 @@ int enqueue_step(nuosc_sim* s, hipEvent_t* ev = nullptr) {
                 launch_open_ghosts<T>(g, y, s->d_g0, s->d_g0 + s->nv, s->inflow, s->d_time, cdt, s->stream);
             if (e != hipSuccess) return e;
         }
-        return launch_xghosts<T>(g, y, s->stream);
+        // (the NOX stage kernel reads no x neighbours: no ghost rows needed)
+        return g.nox ? hipSuccess : launch_xghosts<T>(g, y, s->stream);
     };
     auto mark = [&](int k) { return ev ? hipEventRecord(ev[k], s->stream) : hipSuccess; };
     HIPCK(s, ghosts(u, 0.0));
@@ -317,7 +318,8 @@ int enqueue_step_srk3(nuosc_sim* s) {
                 launch_open_ghosts<T>(g, y, s->d_g0, s->d_g0 + s->nv, s->inflow, s->d_time, cdt, s->stream);
             if (e != hipSuccess) return e;
         }
-        return launch_xghosts<T>(g, y, s->stream);
+        // (the NOX stage kernel reads no x neighbours: no ghost rows needed)
+        return g.nox ? hipSuccess : launch_xghosts<T>(g, y, s->stream);
     };
     HIPCK(s, ghosts(u, 0.0));
     HIPCK(s, launch_stage<T>(5, g, s->ph, s->plan, u, nullptr, nullptr, ya, s->mom_u, P, 0.0, dt, s->stream));
@@ -411,7 +413,7 @@ hipError_t enqueue_scratch_stages(nuosc_sim* s) {
     auto ghosts = [&](T* y, double cdt) {
         if (e == hipSuccess && open)
             e = launch_open_ghosts<T>(g, y, s->d_g0, s->d_g0 + s->nv, s->inflow, s->d_time, cdt, s->stream);
-        if (e == hipSuccess) e = launch_xghosts<T>(g, y, s->stream);
+        if (e == hipSuccess && !g.nox) e = launch_xghosts<T>(g, y, s->stream);
     };
     auto run = [&](hipError_t r) { if (e == hipSuccess) e = r; };
     ghosts(u, 0.0);
@@ -591,6 +593,8 @@ int nuosc_create(const nuosc_config* cfg, nuosc_sim** out) {
     s->geo.nzs = nzs;
     s->geo.zoff = zoff;
     s->geo.nf = s->nf;
+    // one x row with the default discretisation: the stage kernel's NOX form
+    s->geo.nox = (s->nx == 1 && s->nf == kNVar && c.ko_order != 5 && c.adv == NUOSC_ADV_CFD4) ? 1 : 0;
     s->inflow = Inflow{0, 0.0, 1.0, c.zc, c.z0, c.dz};
     if (c.bc == NUOSC_BC_OPEN_Z) {
         if (hipMalloc((void**)&s->d_time, sizeof(double)) != hipSuccess) return bail(NUOSC_ENOMEM, "allocation failed");
@@ -1178,8 +1182,9 @@ int batch_tables(nuosc_batch* b) {
     for (nuosc_sim* m : b->members) {
         BatchMember bm;
         bm.g = m->geo;
+        bm.g.nox = 0;   // the batched stage kernel has no NOX form
         bm.ph = m->ph;
-        bm.pl = b->zlen > 0 ? plan_stage_zlen(m->geo, std::min(b->zlen, m->nz)) : m->plan;
+        bm.pl = plan_stage_zlen(bm.g, b->zlen > 0 ? std::min(b->zlen, m->nz) : m->plan.zlen);
         bm.u = m->st_u;
         bm.w = m->st_w;
         bm.ya = m->st_ya;

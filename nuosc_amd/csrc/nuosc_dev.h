@@ -73,6 +73,8 @@ struct Geometry {
     int bc;                // 0 periodic, 1 open in z
     int nzs, zoff;         // stored z rows per x row, stored row of interior z = 0
     int nf;                // fields per site record: 8 (2 flavours) or 18 (3 flavours)
+    int nox;               // one x row (nx = 1) and the default discretisation: the stage
+                           // kernel's NOX form (x neighbours are the point itself)
 };
 
 // k mod n in 0 .. n-1

@@ -357,7 +357,13 @@ __device__ __forceinline__ void block_sync() {
 // chunk, z segment), quad group fastest; wave wib of the block takes quad
 // W * group + wib.  Lane (g, dl) of a wave handles x row i0 + g and the V
 // adjacent directions ch * D + dl * V + e, e < V.
-template <typename T, int R, int W, int RAD, int STAGE, bool COMP>
+// NOX (grids of one x row): the x neighbours of every point are its own
+// value (the periodic ghost rows are copies of row 0), so no LDS slab, edge
+// rows or barrier are needed, and the block's W R row groups work on W R
+// consecutive direction chunks of row 0 instead of W R rows of one chunk.
+// The x terms are still evaluated, from the centre value, as the reference
+// does with its ghost copies.
+template <typename T, int R, int W, int RAD, int STAGE, bool COMP, bool NOX = false>
 __device__ __forceinline__ void stage_body(const StageArgs<T>& a, int lblock, int wib, Slab<T, R, W, RAD>& slab) {
     constexpr int NZW = 2 * RAD + 1;   // z window
     constexpr int V = kVals<T>;    // directions per lane (one 8-byte load)
@@ -381,14 +387,16 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& a, int lblock, in
     const int nqg = (a.nquads + W - 1) / W;
     const int ib = (lblock % nqg) * W * R;   // first x row of the block
     const int rest = lblock / nqg;
-    const int ch = rest % nch;
-    const int zs = rest / nch;
+    const int nchb = NOX ? (nch + W * R - 1) / (W * R) : nch;   // chunk groups of the block grid
+    const int chb = rest % nchb;
+    const int zs = rest / nchb;
     // (a wave whose quad lies past the last one still runs, with all its
     // rows past nx: every wave of the block takes part in each step's barrier)
-    const int i0 = ib + wib * R;
+    const int i0 = NOX ? 0 : ib + wib * R;
     const int g = lane / L, dl = lane % L;
+    const int ch = NOX ? chb * W * R + wib * R + g : chb;   // the lane's direction chunk
     const int p = ch * L + dl;           // direction group: directions p*V .. p*V+V-1
-    const int r = i0 + g;
+    const int r = NOX ? 0 : i0 + g;
     const bool row_ok = r < nx;
     const bool act = row_ok && p * V < nv;   // nv is a multiple of V
     const int pl = min(p, nv / V - 1);       // loads stay inside the directions
@@ -408,7 +416,7 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& a, int lblock, in
     // rows hold jj in [-zoff, nz + zoff))
     auto zrow = [&](int jj) { return (unsigned)wrapi(jj + zoff, nzs); };
     const unsigned dof = (unsigned)(pl * V) * sizeof(T);
-    const int own = wib * R + g;             // block row of the lane
+    const int own = NOX ? 0 : wib * R + g;   // block row of the lane
     const unsigned vo = (unsigned)min(own + kXg, spare) * xrowb + dof;
     // Stores are unconditional (no branch, so the compiler can count the
     // outstanding memory operations across iterations exactly): a lane past
@@ -416,7 +424,11 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& a, int lblock, in
     // reads; a lane past nv works on the last direction (pl) throughout and
     // stores the same bits to the same address as that direction's lane, so
     // the padding columns are never written.
-    const unsigned vos = (unsigned)(row_ok ? own + kXg : spare) * xrowb + dof;
+    // (NOX: lanes of chunks past the last one sit in other waves than the
+    // owner of direction nv-1 and store into the spare row: the SSP-RK3
+    // stage 7 rewrites u in place, so a duplicate store could reach u before
+    // the owner has read it)
+    const unsigned vos = (unsigned)(row_ok && (!NOX || ch < nch) ? own + kXg : spare) * xrowb + dof;
     // Edge rows: edge e = wib + 4g (row group g < 2 of wave wib, e < 2 RAD)
     // loads block row e - RAD (e < RAD) or WR + e - RAD with its L lanes.  The
     // other lanes' offset lies past the descriptor's range, so they fetch
@@ -500,7 +512,7 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& a, int lblock, in
     }
     // edge row of z = j0 straight into its level; z = j0+1 into hreg (staged
     // in the first step)
-    {
+    if constexpr (!NOX) {
         const unsigned so = zrow(j0) * siteb, s1 = zrow(j0 + 1) * siteb;
 #pragma unroll
         for (int f = 0; f < kNVar; ++f) {
@@ -517,19 +529,21 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& a, int lblock, in
     for (int j = j0; j < j1; ++j) {
         const unsigned so = (unsigned)(j + zoff) * siteb;
         const int lv = j & 1;
-        // own rows of z = j into this level (the edge rows of z = j are there)
+        if constexpr (!NOX) {
+            // own rows of z = j into this level (the edge rows of z = j are there)
 #pragma unroll
-        for (int f = 0; f < kNVar; ++f)
+            for (int f = 0; f < kNVar; ++f)
 #pragma unroll
-            for (int e = 0; e < V; ++e) xs[lv][f][slot_c + e] = win[f][RAD][e];
-        // all rows of z = j are in level lv, and every wave is done with step
-        // j-1, whose level receives the edge rows of z = j+1 (loaded in step
-        // j-1; no wave reads that level before the next barrier)
-        block_sync();
+                for (int e = 0; e < V; ++e) xs[lv][f][slot_c + e] = win[f][RAD][e];
+            // all rows of z = j are in level lv, and every wave is done with step
+            // j-1, whose level receives the edge rows of z = j+1 (loaded in step
+            // j-1; no wave reads that level before the next barrier)
+            block_sync();
 #pragma unroll
-        for (int f = 0; f < kNVar; ++f)
+            for (int f = 0; f < kNVar; ++f)
 #pragma unroll
-            for (int e = 0; e < V; ++e) xs[lv ^ 1][f][slot_h + e] = hreg[f][e];
+                for (int e = 0; e < V; ++e) xs[lv ^ 1][f][slot_h + e] = hreg[f][e];
+        }
         // ---- the step's loads, in the order they are consumed: own row
         // z+RAD (stencil), u and w (update), edge row z+2 (staged in the next
         // step); the moments of z+1 are loaded at the end of the step
@@ -553,9 +567,11 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& a, int lblock, in
             for (int f = 0; f < kNVar; ++f) ww.template ldv<2>(bw, f, vo, so, wc[f]);
             __builtin_amdgcn_sched_barrier(0);
         }
+        if constexpr (!NOX) {
 #pragma unroll
-        for (int f = 0; f < kNVar; ++f) yin.ldv(by, f, vh, s2, hreg[f]);
-        __builtin_amdgcn_sched_barrier(0);
+            for (int f = 0; f < kNVar; ++f) yin.ldv(by, f, vh, s2, hreg[f]);
+            __builtin_amdgcn_sched_barrier(0);
+        }
 
         T yo[kNVar][V], wn[kNVar][V];
 #pragma unroll
@@ -584,8 +600,10 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& a, int lblock, in
                 const T c0 = win[f][RAD][e];
                 T adv, ko;
                 if constexpr (RAD == 2) {   // the reference's default discretisation
-                    const T xm2 = xs[lv][f][slot_c - 2 * D + e], xm1 = xs[lv][f][slot_c - D + e];
-                    const T xp1 = xs[lv][f][slot_c + D + e], xp2 = xs[lv][f][slot_c + 2 * D + e];
+                    const T xm2 = NOX ? c0 : xs[lv][f][slot_c - 2 * D + e];
+                    const T xm1 = NOX ? c0 : xs[lv][f][slot_c - D + e];
+                    const T xp1 = NOX ? c0 : xs[lv][f][slot_c + D + e];
+                    const T xp2 = NOX ? c0 : xs[lv][f][slot_c + 2 * D + e];
                     const T zm2 = win[f][0][e], zm1 = win[f][1][e], zp1 = win[f][3][e], zp2 = win[f][4][e];
                     // advection (nuosc_class.cpp:182-183)
                     adv = fz[e] * ((zm2 - zp2) - (T)8.0 * (zm1 - zp1)) + fx[e] * ((xm2 - xp2) - (T)8.0 * (xm1 - xp1));
@@ -596,8 +614,8 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& a, int lblock, in
                     T xm[4], xp[4], zm[4], zp[4];
 #pragma unroll
                     for (int kk = 1; kk <= RAD; ++kk) {
-                        xm[kk] = xs[lv][f][slot_c - kk * D + e];
-                        xp[kk] = xs[lv][f][slot_c + kk * D + e];
+                        xm[kk] = NOX ? c0 : xs[lv][f][slot_c - kk * D + e];
+                        xp[kk] = NOX ? c0 : xs[lv][f][slot_c + kk * D + e];
                         zm[kk] = win[f][RAD - kk][e];
                         zp[kk] = win[f][RAD + kk][e];
                     }
@@ -680,8 +698,11 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& a, int lblock, in
                 __builtin_amdgcn_make_buffer_rsrc((void*)bp, 0, (int)a.mpart_bytes, kRsrcWord3);
             const int sv = dl >> sh;                       // value index 0..15
             const bool used = sv < kNMom;
-            const unsigned rec = ((unsigned)((row_ok && used ? r : nx) - ib) * (unsigned)nz * nch + (unsigned)ch) *
-                                 (kMomRec * 8);
+            // (NOX: row groups past the last chunk write the spare row's records)
+            const bool rec_ok = row_ok && used && (!NOX || ch < nch);
+            const unsigned rec =
+                ((unsigned)((rec_ok ? r : nx) - ib) * (unsigned)nz * nch + (unsigned)(NOX ? min(ch, nch - 1) : ch)) *
+                (kMomRec * 8);
             const unsigned srec = (mbase + j) * (unsigned)(nch * kMomRec * 8);
             Lane<double>::st(tot, rp, rec + (unsigned)(used ? sv : sv - kNMom) * 8, srec);
         }
@@ -710,12 +731,12 @@ constexpr int stage_waves() {
 
 // RAD = 2: the reference's default discretisation; RAD = 3: any of the
 // schemes, chosen at run time (Geometry::ko5, Geometry::adv)
-template <typename T, int R, int STAGE, bool COMP, int RAD = 2>
+template <typename T, int R, int STAGE, bool COMP, int RAD = 2, bool NOX = false>
 __global__ __launch_bounds__(64 * StageShape<T>::W) __attribute__((amdgpu_waves_per_eu(stage_waves<T, STAGE, RAD>()))) void k_stage(const StageArgs<T> a) {
     constexpr int W = StageShape<T>::W;
-    __shared__ Slab<T, R, W, RAD> slab;
+    __shared__ Slab<T, R, W, RAD> slab;   // (never touched by NOX, which the compiler then leaves unallocated)
     const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    stage_body<T, R, W, RAD, STAGE, COMP>(a, xcd_remap(blockIdx.x, gridDim.x), wib, slab);
+    stage_body<T, R, W, RAD, STAGE, COMP, NOX>(a, xcd_remap(blockIdx.x, gridDim.x), wib, slab);
 }
 
 // Many independent runs in one launch: logical block -> (run, block of run)
@@ -746,7 +767,9 @@ StagePlan plan_stage(const Geometry& g, int num_cus) {
     StagePlan p{};
     const int R = g.rows, W = g.wpb;
     const long long quads = (g.nx + R - 1) / R;
-    const long long rows = quads * g.nch;
+    // NOX: the blocks of one z segment cover the chunks in groups of W R
+    const long long cgroups = g.nox ? (g.nch + W * R - 1) / (W * R) : g.nch;
+    const long long rows = g.nox ? cgroups * W : quads * g.nch;
     const long long want = (long long)num_cus * 48;
     int zlen = g.nz;
     while (zlen > 8 && rows * ((g.nz + zlen - 1) / zlen) < want) zlen = (zlen + 1) / 2;
@@ -754,7 +777,7 @@ StagePlan plan_stage(const Geometry& g, int num_cus) {
     p.zlen = zlen;
     p.nseg = (g.nz + zlen - 1) / zlen;
     p.nquads = (int)quads;
-    p.nblocks = (int)(((quads + W - 1) / W) * g.nch * p.nseg);
+    p.nblocks = (int)(((quads + W - 1) / W) * cgroups * p.nseg);
     p.ntask = p.nblocks * W;
     return p;
 }
@@ -763,10 +786,11 @@ StagePlan plan_stage_zlen(const Geometry& g, int zlen) {
     StagePlan p{};
     const int R = g.rows, W = g.wpb;
     const long long quads = (g.nx + R - 1) / R;
+    const long long cgroups = g.nox ? (g.nch + W * R - 1) / (W * R) : g.nch;
     p.zlen = std::max(1, std::min(zlen, g.nz));
     p.nseg = (g.nz + p.zlen - 1) / p.zlen;
     p.nquads = (int)quads;
-    p.nblocks = (int)(((quads + W - 1) / W) * g.nch * p.nseg);
+    p.nblocks = (int)(((quads + W - 1) / W) * cgroups * p.nseg);
     p.ntask = p.nblocks * W;
     return p;
 }
@@ -822,7 +846,19 @@ hipError_t launch_stage(int stage, const Geometry& g, const Physics& ph, const S
     constexpr int R = StageShape<T>::R;
     if (g.lpr != dirs_per_chunk((int)sizeof(T)) || g.wpb != StageShape<T>::W) return hipErrorInvalidValue;
     dim3 grid(pl.nblocks), block(64 * StageShape<T>::W);
-    if (scheme_radius(g.ko5, g.adv) == 2 && g.adv == kAdvCfd4) {
+    if (g.nox) {   // one x row, default discretisation
+        switch (stage) {
+            case 0: hipLaunchKernelGGL((k_stage<T, R, 0, COMP, 2, true>), grid, block, 0, s, a); break;
+            case 1: hipLaunchKernelGGL((k_stage<T, R, 1, COMP, 2, true>), grid, block, 0, s, a); break;
+            case 2: hipLaunchKernelGGL((k_stage<T, R, 2, COMP, 2, true>), grid, block, 0, s, a); break;
+            case 3: hipLaunchKernelGGL((k_stage<T, R, 3, COMP, 2, true>), grid, block, 0, s, a); break;
+            case 4: hipLaunchKernelGGL((k_stage<T, R, 4, COMP, 2, true>), grid, block, 0, s, a); break;
+            case 5: hipLaunchKernelGGL((k_stage<T, R, 5, COMP, 2, true>), grid, block, 0, s, a); break;
+            case 6: hipLaunchKernelGGL((k_stage<T, R, 6, COMP, 2, true>), grid, block, 0, s, a); break;
+            case 7: hipLaunchKernelGGL((k_stage<T, R, 7, COMP, 2, true>), grid, block, 0, s, a); break;
+            default: return hipErrorInvalidValue;
+        }
+    } else if (scheme_radius(g.ko5, g.adv) == 2 && g.adv == kAdvCfd4) {
         switch (stage) {
             case 0: hipLaunchKernelGGL((k_stage<T, R, 0, COMP>), grid, block, 0, s, a); break;
             case 1: hipLaunchKernelGGL((k_stage<T, R, 1, COMP>), grid, block, 0, s, a); break;
@@ -1053,7 +1089,8 @@ hipError_t batch_build(BatchTables& t, const std::vector<BatchMember>& ms, int e
     if (ms.empty()) return hipSuccess;
     for (const auto& m : ms) {
         if (!buffer_addressable(m.g, esize)) return hipErrorInvalidValue;
-        if (m.g.lpr != dirs_per_chunk(esize) || m.g.wpb != waves_per_block(esize)) return hipErrorInvalidValue;
+        if (m.g.lpr != dirs_per_chunk(esize) || m.g.wpb != waves_per_block(esize) || m.g.nox)
+            return hipErrorInvalidValue;
     }
     return esize == 8 ? batch_build_t<double>(t, ms, s) : batch_build_t<float>(t, ms, s);
 }

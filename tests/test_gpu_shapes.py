@@ -11,6 +11,8 @@ quad into the last block, periodic ghost rows that wrap more than once (nx <
     FP32: within 64 eps32 max|field| / h per field (FP32 rounding of the
     stencil);
   * 30 RK4 steps against the oracle, FP64: relative L-inf <= 1e-10 per field.
+  * one x row: the stage kernel's NOX form (alone) and its ordinary form (in a
+    batch) give the same bits over 20 RK4 steps.
 """
 import numpy as np
 import pytest
@@ -94,3 +96,27 @@ def test_rk4_shapes_30_steps(oracle_mod, nuosc_mod, nx, nz):
         assert rel_linf(gs[f], os_[f]) <= 1e-10, (f, rel_linf(gs[f], os_[f]))
     g.close()
     o.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_one_row_form_bitwise(nuosc_mod, prec):
+    """A grid of one x row runs the stage kernel's NOX form alone (x
+    neighbours taken as the point itself) and the ordinary form inside a
+    batch (x neighbours from the periodic ghost rows, copies of the row):
+    the two give the same bits."""
+    N = nuosc_mod
+    p = N.PREC_FP64 if prec == "fp64" else N.PREC_FP32
+    c = cfg_for(1, 40, 4, pmo=0.5)
+    init = dict(ipt=5, alpha=0.9, eps0=0.1, sigma=1.0, lnue=0.6, lnueb=0.5, seed=3)
+    a = N.NuOsc(**c, precision=p)
+    b = N.NuOsc(**c, precision=p)
+    for g in (a, b):
+        g.fillInitValue(**init)
+    bt = N.Batch(precision=p)
+    bt.add(b)
+    a.step_rk4(20)
+    bt.step(20)
+    assert np.array_equal(a.get_state(), b.get_state())
+    bt.close()
+    a.close()
+    b.close()
