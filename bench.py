@@ -21,6 +21,8 @@ Prints one JSON line:
   cpu_baseline   the CPU oracle on a 128 x 128 x 642 crop of the same physics,
                  timed on this host's cores (a reported baseline)
 
+--shape column runs the 1-D open column (1 x 4096 cells x 642 directions,
+gz=3, an ipt-0 pulse entering through z0) instead of the box.
 --nflavor 3 runs the same box with 3 flavours (18 fields, pmo=1): the same
 12 passes per step, of 18 fields; the roofline line is the stage-2 kernel
 (4 passes of 18 fields).
@@ -61,8 +63,8 @@ def parse():
                         "served: Poisson arrivals into the run server; precision: the FP32 form against FP64 "
                         "from the same initial state (--shape box|column)")
     p.add_argument("--shape", default="box", choices=("box", "column"),
-                   help="precision: the 512x512x642 periodic box (seeded eps=1e-6) or the 1-D open column "
-                        "(4096 cells in z, 642 directions, gz=3, an ipt-0 pulse entering through z0)")
+                   help="box and precision workloads: the 512x512x642 periodic box (seeded eps=1e-6) or the 1-D "
+                        "open column (4096 cells in z, 642 directions, gz=3, an ipt-0 pulse entering through z0)")
     p.add_argument("--runs", type=int, default=64)
     p.add_argument("--grids", default=None,
                    help="batched/served: comma-separated square grid sizes the runs are drawn from "
@@ -362,10 +364,18 @@ def main():
     nvar = 18 if three else 8
     passes = STAGE_PASSES
     pmo = 1.0 if three else 0.0
-    cfg = dict(nv=a.nn, x0=-a.zmax, x1=a.zmax, z0=-a.zmax, z1=a.zmax, dx=a.dz, dz=a.dz, CFL=0.4, ko=0.1,
-               mu=1.0, pmo=pmo, precision=prec, use_graph=True, nflavor=a.nflavor)
+    column = a.shape == "column"
+    if column:   # the 1-D open column of the precision workload
+        cfg = dict(nv=a.nn, x0=-0.025, x1=0.025, z0=-102.4, z1=102.4, dx=0.05, dz=0.05, CFL=0.4, ko=0.1, mu=1.0,
+                   pmo=pmo, gz=3, bc=N.BC_OPEN_Z, zc=-105.4, precision=prec, use_graph=True, nflavor=a.nflavor)
+    else:
+        cfg = dict(nv=a.nn, x0=-a.zmax, x1=a.zmax, z0=-a.zmax, z1=a.zmax, dx=a.dz, dz=a.dz, CFL=0.4, ko=0.1,
+                   mu=1.0, pmo=pmo, precision=prec, use_graph=True, nflavor=a.nflavor)
     g = N.NuOsc(device=0, **cfg)
-    g.fillInitValue(ipt=5, alpha=0.9, eps0=1e-6, sigma=1.0, lnue=0.6, lnueb=0.5, seed=12345)
+    if column:
+        g.fillInitValue(ipt=0, alpha=0.9, eps0=1e-6, sigma=1.0, lnue=0.6, lnueb=0.5)
+    else:
+        g.fillInitValue(ipt=5, alpha=0.9, eps0=1e-6, sigma=1.0, lnue=0.6, lnueb=0.5, seed=12345)
     nx, nz, nv = g.shape
     upd = nx * nz * nv
     zlen = g.plan()[0] if a.no_autotune else g.autotune(trials=2)[0]
@@ -401,9 +411,11 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64" if a.precision == 64 else "f32",
-        "data": "synthetic (seeded eps=1e-6 complex perturbation)",
+        "data": "synthetic (ipt-0 pulse entering through z0)" if column
+                else "synthetic (seeded eps=1e-6 complex perturbation)",
         "config": {
-            "workload": f"{nx}x{nz} periodic box x {nv} icosahedral directions, "
+            "workload": (f"{nx}x{nz} 1-D open column (gz=3) x {nv} icosahedral directions, " if column else
+                         f"{nx}x{nz} periodic box x {nv} icosahedral directions, ") +
                         f"{'3 flavours (18 fields), ' if three else ''}"
                         f"{'FP64' if a.precision == 64 else 'FP32 + FP64 moments'}, mu=1 ko=0.1 cfl=0.4 pmo={pmo:g}",
             "nx": nx, "nz": nz, "nv": nv, "dz": a.dz, "zlen": zlen,
@@ -415,7 +427,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": s2_gbs / HBM_PEAK_GBS,
-            "traffic": None if three else traffic_from_profiles(a.precision),
+            "traffic": None if (three or column) else traffic_from_profiles(a.precision),
             "algorithmic_bytes_per_launch": s2_bytes,
             "launch_ms": stage_ms[1],
         },
