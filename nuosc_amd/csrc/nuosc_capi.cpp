@@ -433,7 +433,7 @@ extern "C" {
 int nuosc_abi_version(void) { return NUOSC_ABI_VERSION; }
 
 const char* nuosc_build_info(void) {
-    return "libnuosc_hip: gfx950 RK4 stage kernels (fp64, fp32 + fp64 moments; periodic and open-z boundaries), "
+    return "libnuosc_hip: gfx950 RK4 stage kernels (fp64, fp32 + fp64 moments; periodic and open-z boundaries; 2 and 3 flavours), "
            "abi 4";
 }
 
