@@ -115,6 +115,8 @@ struct F3Args {
     double mu, pmo;
     double h[9];
     int tiles, quads, nch;
+    int one_row;   // nx = 1: the block's waves take kF3Rows direction chunks of that row
+    int cgroups;   // chunk groups of the block grid (nch, or ceil(nch / kF3Rows) for one row)
 };
 
 // Moment differences (nuosc_class.cpp:57-71) a field's value enters, with
@@ -199,7 +201,7 @@ struct XSlab {
 // instead of 4.  All waves of the block reach every barrier; waves whose row
 // is past nx load a stored row (at most nx + 1, a periodic ghost row) and
 // compute nothing.
-template <typename T, int STAGE>
+template <typename T, int STAGE, bool ONE>
 __device__ __forceinline__ void stage3_body(const F3Args<T>& a, const T* __restrict__ yin, XSlab<T>& sx) {
     constexpr int kGrp = kGrp3<T>;
     const Geometry& g = a.g;
@@ -210,10 +212,17 @@ __device__ __forceinline__ void stage3_body(const F3Args<T>& a, const T* __restr
     const int qx = r % a.quads;
     r /= a.quads;
     const int jt = r % a.tiles;
-    const int ch = r / a.tiles;
+    const int chg = r / a.tiles;
     const int j = jt * kF3Tile + jj;
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    const int i = qx * kF3Rows + w;
+    // one x row (nx = 1): the x neighbours of every point are the point
+    // itself (its periodic ghost rows are copies of it), so the waves of a
+    // block take kF3Rows consecutive direction chunks of the row instead of
+    // kF3Rows rows of one chunk; the x terms are still evaluated, from the
+    // centre value, and the LDS steps below run unused
+    constexpr bool one = ONE;
+    const int ch = one ? chg * kF3Rows + w : chg;
+    const int i = one ? 0 : qx * kF3Rows + w;
     const int lane = (int)(threadIdx.x & 63);
     if (j >= g.nz) return;   // whole blocks
     const bool row_ok = i < g.nx;
@@ -282,8 +291,8 @@ __device__ __forceinline__ void stage3_body(const F3Args<T>& a, const T* __restr
                 const int f = g0 + q;
                 const T* p = yin + o + (long long)f * g.nvp;
                 const T zm2 = p[zo[0]], zm1 = p[zo[1]], zp1 = p[zo[2]], zp2 = p[zo[3]];
-                const T xm2 = sx.x[q][w][lane], xm1 = sx.x[q][w + 1][lane];
-                const T xp1 = sx.x[q][w + 3][lane], xp2 = sx.x[q][w + 4][lane];
+                const T xm2 = one ? c[f] : sx.x[q][w][lane], xm1 = one ? c[f] : sx.x[q][w + 1][lane];
+                const T xp1 = one ? c[f] : sx.x[q][w + 3][lane], xp2 = one ? c[f] : sx.x[q][w + 4][lane];
                 const T c0 = c[f];
                 const T adv =
                     fz * ((zm2 - zp2) - (T)8.0 * (zm1 - zp1)) + fx * ((xm2 - xp2) - (T)8.0 * (xm1 - xp1));
@@ -343,17 +352,17 @@ __device__ __forceinline__ void stage3_body(const F3Args<T>& a, const T* __restr
 #pragma unroll
             for (int q = 0; q < 4; ++q) t[4 * cc + q] = q4[q] * d[cc];
         const double tot = wave_sum32(t, lane);
-        if ((lane & 1) == 0)
+        if ((lane & 1) == 0 && (!ONE || ch < a.nch))   // (one row: waves past the last chunk add nothing)
             a.part[(((long long)i * g.nz + j) * a.nch + ch) * kNMom3 + (lane >> 1)] = tot;
     }
 }
 
 // 4 waves per SIMD (128 VGPRs; FP64 stages 2-4, 6, 7 spill 2 registers):
 // measured 7% faster per FP64 step than the 3 waves the compiler picks alone
-template <typename T, int STAGE>
+template <typename T, int STAGE, bool ONE>
 __global__ __launch_bounds__(64 * kF3Rows) __attribute__((amdgpu_waves_per_eu(4))) void k_stage3(const F3Args<T> a) {
     __shared__ XSlab<T> sx;
-    stage3_body<T, STAGE>(a, a.yin, sx);
+    stage3_body<T, STAGE, ONE>(a, a.yin, sx);
 }
 
 // per-site totals of the partial records, summed in chunk order
@@ -370,7 +379,8 @@ __global__ __launch_bounds__(256) void k_moment_totals3(const double* __restrict
 
 template <typename T, int STAGE>
 hipError_t launch3(const F3Args<T>& a, unsigned nb, hipStream_t s) {
-    hipLaunchKernelGGL((k_stage3<T, STAGE>), dim3(nb), dim3(64 * kF3Rows), 0, s, a);
+    if (a.one_row) hipLaunchKernelGGL((k_stage3<T, STAGE, true>), dim3(nb), dim3(64 * kF3Rows), 0, s, a);
+    else hipLaunchKernelGGL((k_stage3<T, STAGE, false>), dim3(nb), dim3(64 * kF3Rows), 0, s, a);
     return hipGetLastError();
 }
 
@@ -415,7 +425,9 @@ hipError_t launch_stage3(int stage, const Geometry& g, const Physics& ph, const 
     for (int q = 0; q < 9; ++q) a.h[q] = ph.h3[q];
     a.tiles = (g.nz + kF3Tile - 1) / kF3Tile;
     a.quads = (g.nx + kF3Rows - 1) / kF3Rows;
-    const long long nb = (long long)a.nch * a.tiles * a.quads * kF3Tile;
+    a.one_row = g.nx == 1;
+    a.cgroups = a.one_row ? (a.nch + kF3Rows - 1) / kF3Rows : a.nch;
+    const long long nb = (long long)a.cgroups * a.tiles * a.quads * kF3Tile;
     if (nb <= 0 || nb > 0x7fffffffLL) return hipErrorInvalidValue;
     switch (stage) {
         case 0: return launch3<T, 0>(a, (unsigned)nb, s);
