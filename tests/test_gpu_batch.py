@@ -74,6 +74,12 @@ def test_estimate_matches_allocation(nuosc_mod):
             est = N.estimate_device_bytes(precision=p, **c)
             held = g.device_bytes()
             assert est >= held * 0.95 and est <= held * 1.5 + (1 << 20), (est, held)
+            g.close()
+            g3 = N.NuOsc(precision=p, nflavor=3, **c)
+            est3 = N.estimate_device_bytes(precision=p, nflavor=3, **c)
+            held3 = g3.device_bytes()
+            assert est3 >= held3 * 0.95 and est3 <= held3 * 1.5 + (1 << 20), (est3, held3)
+            g3.close()
 
 
 def test_autotune_keeps_state_and_results(nuosc_mod):
