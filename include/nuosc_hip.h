@@ -3,7 +3,7 @@
  * of the nuosc RK4 time-stepper core.
  *
  * The reference exposes this path through the C++ class NuOsc
- * (nuosc_class.h:188-280) and the ./nuosc command line (nuosc.cpp:17-261); it
+ * (nuosc_class.h:99-191) and the ./nuosc command line (nuosc.cpp:17-261); it
  * has no plugin or FFI layer.  This header is the plain-C boundary beneath a
  * NuOsc-compatible host class (nuosc_amd/csrc/NuOsc.h) and the ctypes binding
  * (nuosc_amd/__init__.py).  Every entry point below cites the reference
@@ -17,7 +17,7 @@
  *     nuosc_last_error(NULL) for errors of nuosc_create).  No C++ exception
  *     crosses this boundary.
  *   - Host arrays are FP64.  Field order is the reference's
- *     (nuosc_class.h:147-153): 0 ee, 1 mm (=xx), 2 emr, 3 emi, 4 bee, 5 bmm,
+ *     (getAllFields, nuosc_class.h:58-65): 0 ee, 1 mm (=xx), 2 emr, 3 emi, 4 bee, 5 bmm,
  *     6 bemr, 7 bemi.  The "interior" layout is [nx][nz][nv] (v fastest); the
  *     "ghosted" layout is the reference's lpts layout
  *     ((i+gx)*(nz+2gz)+j+gz)*nv+v (CartGrid.h:91).
@@ -82,9 +82,9 @@ extern "C" {
 typedef struct nuosc_sim nuosc_sim;
 
 /* Run configuration.  Replaces the NuOsc constructor arguments
- * NuOsc(px,pz,nv,nphi,gx,gz,x0,x1,z0,z1,dx,dz,CFL,ko) (nuosc_class.h:220-222,
+ * NuOsc(px,pz,nv,nphi,gx,gz,x0,x1,z0,z1,dx,dz,CFL,ko) (nuosc_class.h:131-133,
  * nuosc_class.cpp:454-466) plus set_mu/set_pmo/set_renorm
- * (nuosc_class.h:237-248).  px,pz (MPI) and nphi (polar grid only) have no
+ * (nuosc_class.h:148-159).  px,pz (MPI) and nphi (polar grid only) have no
  * counterpart here. */
 typedef struct nuosc_config {
     int nn;          /* icosahedron refinement (--nv): N = 10*nn^2+2 directions */
@@ -151,17 +151,17 @@ int nuosc_vgrid(int nn, int mode, double* vx, double* vy, double* vz, double* vw
 /* NuOsc::NuOsc (nuosc_class.cpp:454-534): geometry, velocity grid,
  * allocation.  The state is zero until nuosc_fill_init or an upload. */
 int nuosc_create(const nuosc_config* cfg, nuosc_sim** out);
-/* NuOsc::~NuOsc (nuosc_class.h:225-235) */
+/* NuOsc::~NuOsc (nuosc_class.h:136-146) */
 int nuosc_destroy(nuosc_sim* sim);
 const char* nuosc_last_error(const nuosc_sim* sim);
 
-/* NuOsc::set_mu / set_pmo / set_renorm (nuosc_class.h:237-248) */
+/* NuOsc::set_mu / set_pmo / set_renorm (nuosc_class.h:148-159) */
 int nuosc_set_mu(nuosc_sim* sim, double mu);
 int nuosc_set_pmo(nuosc_sim* sim, double pmo);
 int nuosc_set_renorm(nuosc_sim* sim, int renorm);
 
 /* Grid queries: CartGrid members nx, nz, nv, lpts (CartGrid.h:27-34,85-89)
- * and NuOsc members dt, ds_L, phy_time, n_nue0 (nuosc_class.h:192-203). */
+ * and NuOsc members dt, ds_L, phy_time, n_nue0 (nuosc_class.h:103-114). */
 int nuosc_get_dims(const nuosc_sim* sim, int* nx, int* nz, int* nv, long long* lpts);
 int nuosc_get_scalars(const nuosc_sim* sim, double* dt, double* ds_L, double* phy_time,
                       double* n_nue0_2);
@@ -171,7 +171,7 @@ int nuosc_set_phy_time(nuosc_sim* sim, double phy_time);
 int nuosc_get_vgrid(const nuosc_sim* sim, double* vx, double* vy, double* vz, double* vw);
 int nuosc_get_coords(const nuosc_sim* sim, double* X, double* Z);
 /* G0, G0b per direction (the reference stores them per point,
- * nuosc_class.h:202). */
+ * nuosc_class.h:113). */
 int nuosc_get_g0(const nuosc_sim* sim, double* G0, double* G0b);
 /* Device bytes held by the handle. */
 int nuosc_get_device_bytes(const nuosc_sim* sim, long long* bytes);

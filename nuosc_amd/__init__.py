@@ -1,7 +1,7 @@
 """nuosc_amd -- MI355X (gfx950) implementation of the nuosc RK4 time-stepper core.
 
 Python binding (ctypes) of the C ABI in include/nuosc_hip.h.  ``NuOsc`` mirrors
-the reference's ``NuOsc`` class (nuosc_class.h:188-280): same constructor
+the reference's ``NuOsc`` class (nuosc_class.h:99-191): same constructor
 arguments, ``set_mu``/``set_pmo``/``set_renorm``, ``fillInitValue``,
 ``step_rk4``, ``calRHS``, ``analysis``, ``renormalize`` and snapshot slices.
 
@@ -180,7 +180,7 @@ def vgrid(nn: int, mode: int = VGRID_VORONOI):
 
 class NuOsc:
     """GPU NuOsc.  Constructor follows NuOsc(px,pz,nv,nphi,gx,gz,x0,x1,z0,z1,dx,dz,CFL,ko)
-    (nuosc_class.h:220-222) without px, pz, nphi; ``nv`` is the icosahedron level."""
+    (nuosc_class.h:131-133) without px, pz, nphi; ``nv`` is the icosahedron level."""
 
     def __init__(self, nv=4, gx=2, gz=2, x0=-1.0, x1=1.0, z0=-10.0, z1=10.0, dx=0.1, dz=0.1,
                  CFL=0.4, ko=0.0, mu=1.0, pmo=0.1, renorm=False, precision=PREC_FP64,
@@ -234,7 +234,7 @@ class NuOsc:
     def shape(self):
         return (self.nx, self.nz, self.nv)
 
-    # -- reference setters (nuosc_class.h:237-248) ---------------------------
+    # -- reference setters (nuosc_class.h:148-159) ---------------------------
     def set_mu(self, mu):
         self._check(lib().nuosc_set_mu(self._h, mu))
 

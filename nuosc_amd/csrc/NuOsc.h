@@ -1,5 +1,5 @@
 // NuOsc.h -- host C++ class with the interface of the reference's NuOsc
-// (nuosc_class.h:188-280), backed by libnuosc_hip.so through the C ABI of
+// (nuosc_class.h:99-191), backed by libnuosc_hip.so through the C ABI of
 // include/nuosc_hip.h.  The state lives on the GPU; host code sees copies.
 //
 // Kept: constructor arguments, set_mu / set_pmo / set_renorm, get_lpts /

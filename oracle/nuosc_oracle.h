@@ -92,7 +92,7 @@ typedef struct orc_sim {
     double phy_time;
     double n_nue0[2];
     double *G0, *G0b;          /* per direction (the reference stores them per point) */
-    /* four RK4 state buffers, 8 fields each (nuosc_class.h:106-156) */
+    /* four RK4 state buffers, nvar (8 or 18) fields each (nuosc_class.h:108-109) */
     double *stat[ORC_NVAR], *rhs[ORC_NVAR], *pre[ORC_NVAR], *cor[ORC_NVAR];
     int analysis_lines;        /* analysis.dat line counter (first line precision quirk) */
     /* boundary: 0 periodic (reference), 1 open in z (x periodic) */
