@@ -4,7 +4,9 @@ Bars (stated per test):
   * setup (velocity grid, G0, deterministic initial data): bit-exact;
   * one calRHS: relative L-inf <= 1e-12 per field (only the reduction order
     of the velocity moments and fused multiply-adds differ);
-  * 200 RK4 steps of the 64x64x162 box: relative L-inf <= 1e-10 per field,
+  * 200 RK4 steps of the 64x64x162 box (the ipt-0 pulse on Voronoi weights,
+    and the seeded complex eps = 1e-6 perturbation on equal weights):
+    relative L-inf <= 1e-10 per field,
     Pee and |M0| series relative 1e-10, |P|-1 and ELNe absolute 1e-12 +
     relative 1e-10 (they sit at the round-off level).
 """
@@ -104,13 +106,19 @@ def test_calrhs(oracle_mod, nuosc_mod, pmo, ko, mu):
         assert rel_linf(rg[f], ro[f]) <= 1e-12, (f, rel_linf(rg[f], ro[f]))
 
 
-@pytest.mark.parametrize("use_graph", [True, False])
-def test_rk4_box64_200_steps(oracle_mod, nuosc_mod, use_graph):
+# (use_graph, init): the reference's ipt-0 pulse on Voronoi weights, and the
+# seeded complex eps = 1e-6 perturbation on equal weights (sigma_nu = 0.6,
+# sigma_nubar = 0.5 as lnue/lnueb, seed 12345)
+SEEDED_EQUAL = dict(ipt=5, alpha=0.9, eps0=1e-6, sigma=1.0, lnue=0.6, lnueb=0.5, seed=12345)
+
+
+@pytest.mark.parametrize("use_graph,init,vgrid_mode", [(True, INIT0, 0), (False, INIT0, 0), (True, SEEDED_EQUAL, 1)])
+def test_rk4_box64_200_steps(oracle_mod, nuosc_mod, use_graph, init, vgrid_mode):
     O, N = oracle_mod, nuosc_mod
-    o = oracle_sim(O, BOX64)
-    g = gpu_sim(N, BOX64, use_graph=use_graph)
-    o.fill_init(**INIT0)
-    g.fillInitValue(**INIT0)
+    o = oracle_sim(O, BOX64, vgrid_mode=vgrid_mode)
+    g = gpu_sim(N, BOX64, use_graph=use_graph, vgrid_mode=vgrid_mode)
+    o.fill_init(**init)
+    g.fillInitValue(**init)
     for step in range(0, 201, 10):
         if step:
             o.step(10)
