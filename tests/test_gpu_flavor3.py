@@ -56,11 +56,12 @@ def full_state(o, seed, amp=0.05):
 SHAPES = [(1, 5), (3, 9), (5, 6), (13, 11), (7, 33)]
 
 
-@pytest.mark.parametrize("nn", [2, 4, 8])
-@pytest.mark.parametrize("nx,nz", SHAPES)
+# 642 directions (nn = 8) on the small shapes only
+CALRHS3_CASES = [(nx, nz, nn) for nx, nz in SHAPES for nn in (2, 4, 8) if nn < 8 or nx * nz <= 60]
+
+
+@pytest.mark.parametrize("nx,nz,nn", CALRHS3_CASES)
 def test_calrhs3_fp64(oracle_mod, nuosc_mod, nx, nz, nn):
-    if nn == 8 and nx * nz > 60:
-        pytest.skip("642 directions: the small shapes suffice")
     O, N = oracle_mod, nuosc_mod
     c = cfg_for(nx, nz, nn)
     o = oracle_sim(O, c)
