@@ -343,7 +343,7 @@ def main():
         copy_calibration()
         return
     if a.gpus != 1:
-        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {a.gpus}); "
+        print(f"bench.py measures one GPU: the library has no multi-GPU path (asked for --gpus {a.gpus}); "
               "the periodic box would shard as an (x,z) domain decomposition with radius-2 halos.",
               file=sys.stderr)
         sys.exit(2)

@@ -33,7 +33,7 @@ NuOsc::NuOsc(int px, int pz, int nv, int nphi, int gx, int gz, double x0, double
     : dx(dx_), dz(dz_), CFL(CFL_), ko(ko_) {
     (void)nphi;  // polar velocity grid only
     if (px != 1 || pz != 1)
-        throw std::runtime_error("this build runs one domain per GPU: --np must be 1 1");
+        throw std::runtime_error("one domain per GPU: --np must be 1 1");
     nuosc_config c{};
     c.nn = nv;
     c.gx = gx;
