@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define NUOSC_ABI_VERSION 4
+#define NUOSC_ABI_VERSION 5
 
 /* status codes */
 #define NUOSC_OK 0
@@ -243,7 +243,7 @@ int nuosc_profile_stages(nuosc_sim* sim, int nsteps, double* stage_ms);
  * rounds after a warm-up round, keeps each candidate's fastest time and
  * selects the best.  Call it at a step boundary: it overwrites only the RK4
  * scratch copies, never the state.  Results are cached per (nx, nz, nv,
- * precision) for the process.  3-flavour handles have no segment length:
+ * precision, bc, ko_order, adv, one-row form) for the process.  3-flavour handles have no segment length:
  * autotune returns NUOSC_OK with *best_zlen = 0 and set_zlen has no effect. */
 int nuosc_get_plan(const nuosc_sim* sim, int* zlen, int* waves);
 int nuosc_set_zlen(nuosc_sim* sim, int zlen);
@@ -256,7 +256,10 @@ int nuosc_autotune(nuosc_sim* sim, int trials, int* best_zlen, double* best_ms);
  * them.  Members keep their own state and remain usable through the
  * single-run functions between batch steps; a member's results are bitwise
  * the same stepped alone or in a batch.  All members share the batch's
- * device and precision; renormalising members are not supported.
+ * device and precision; renormalising members are not supported.  A handle
+ * belongs to at most one batch.  Changing a member's physics or
+ * decomposition (set_mu, set_pmo, set_zlen, autotune) takes effect at the
+ * next batch step; destroying a member removes it from its batch first.
  * ------------------------------------------------------------------------ */
 typedef struct nuosc_batch nuosc_batch;
 int nuosc_batch_create(int device, int precision, nuosc_batch** out);
@@ -283,13 +286,17 @@ int nuosc_estimate_device_bytes(const nuosc_config* cfg, long long* bytes);
 
 /* ------------------------------------------------------------------------
  * Run server (no reference counterpart).  Parameter-scan runs are submitted
- * by any host thread; one dispatcher thread admits them in arrival order
- * while their summed device footprint fits the HBM budget, advances all
+ * by any host thread; one dispatcher thread admits each queued run whose
+ * device footprint fits beside the resident runs' within the HBM budget
+ * (scanning the queue in arrival order, so a small run is not held behind a
+ * large one; once the oldest queued run has waited longer than the age
+ * bound, 30 s by default, no younger run is admitted before it), advances all
  * resident runs together through a batch (continuous batching: runs join and
  * leave between ticks of at most max_tick_steps steps) and records each run's
  * (t, Pee, Peebar, maxdP, |M0|) series every ana_every steps.  submit
- * refuses (NUOSC_EINVAL) runs a batch cannot take: open z, a non-default
- * discretisation, 3 flavours.
+ * refuses (NUOSC_EINVAL) runs a batch cannot take (open z, a non-default
+ * discretisation, 3 flavours, renormalisation) and runs whose footprint
+ * alone exceeds the budget.
  * ------------------------------------------------------------------------ */
 typedef struct nuosc_server nuosc_server;
 
@@ -328,6 +335,9 @@ int nuosc_server_submit(nuosc_server* srv, const nuosc_request* req, long long* 
 int nuosc_server_take_result(nuosc_server* srv, long long id, double timeout_s, nuosc_result* info,
                              double* series, int max_samples);
 int nuosc_server_stats(nuosc_server* srv, nuosc_server_stats_t* stats);
+/* Seconds the oldest queued run may be passed over by younger runs that fit
+ * (>= 0; 0 = strict arrival order). */
+int nuosc_server_set_age_bound(nuosc_server* srv, double seconds);
 /* Stops the dispatcher: queued and resident runs are cancelled. */
 int nuosc_server_destroy(nuosc_server* srv);
 

@@ -145,6 +145,7 @@ SIGNATURES = {
     "nuosc_server_take_result": ([_vp, C.c_longlong, C.c_double, C.POINTER(Result), _dp, C.c_int], C.c_int),
     "nuosc_server_stats": ([_vp, C.POINTER(ServerStats)], C.c_int),
     "nuosc_server_destroy": ([_vp], C.c_int),
+    "nuosc_server_set_age_bound": ([_vp, C.c_double], C.c_int),
 }
 
 
@@ -402,6 +403,10 @@ class Batch:
         self.members.append(sim)
 
     def remove(self, sim):
+        if getattr(sim, "_h", None) is None:   # closed: the library removed it from the batch
+            if sim in self.members:
+                self.members.remove(sim)
+            return
         self._check(lib().nuosc_batch_remove(self._h, sim._h))
         self.members.remove(sim)
 
@@ -479,6 +484,13 @@ class Server:
             raise NuOscError(rc, "take_result failed")
         n = min(info.n_samples, max_samples)
         return info, buf[:5 * n].reshape(n, 5).copy()
+
+    def set_age_bound(self, seconds):
+        """Seconds the oldest queued run may be passed over by younger runs
+        that fit (0: strict arrival order)."""
+        rc = lib().nuosc_server_set_age_bound(self._h, float(seconds))
+        if rc != 0:
+            raise NuOscError(rc, "bad age bound")
 
     def stats(self):
         st = ServerStats()

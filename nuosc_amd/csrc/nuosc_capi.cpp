@@ -68,6 +68,7 @@ struct nuosc_sim {
     Geometry geo{};
     Physics ph{};
     StagePlan plan{};
+    nuosc_batch* batch = nullptr;   // the batch this run is a member of, if any
     std::string err;
 };
 
@@ -78,6 +79,12 @@ int fail(nuosc_sim* s, int code, const std::string& msg) {
     else g_create_error = msg;
     return code;
 }
+
+// A batch member changed what the batch's device tables hold (physics,
+// decomposition): the tables are rebuilt before the next batch step.
+void batch_member_changed(nuosc_batch* b);
+// A member is being destroyed: it leaves its batch first.
+void batch_detach(nuosc_batch* b, nuosc_sim* s);
 
 #define HIPCK(sim, call)                                                                      \
     do {                                                                                      \
@@ -435,7 +442,7 @@ int nuosc_abi_version(void) { return NUOSC_ABI_VERSION; }
 
 const char* nuosc_build_info(void) {
     return "libnuosc_hip: gfx950 RK4 stage kernels (fp64, fp32 + fp64 moments; periodic and open-z boundaries; 2 and 3 flavours), "
-           "abi 4";
+           "abi 5";
 }
 
 int nuosc_vgrid(int nn, int mode, double* vx, double* vy, double* vz, double* vw) {
@@ -626,6 +633,9 @@ int nuosc_create(const nuosc_config* cfg, nuosc_sim** out) {
 
 int nuosc_destroy(nuosc_sim* s) {
     if (!s) return NUOSC_OK;
+    // a member leaves its batch before its buffers go (the batch's tables
+    // and captured step point at them)
+    if (s->batch) batch_detach(s->batch, s);
     if (s->stream) hipStreamSynchronize(s->stream);
     drop_graph(s);
     void* bufs[] = {s->st_u, s->st_w, s->st_ya, s->st_yb, s->mom_part, s->mom_u, s->mom_a, s->mom_b,
@@ -644,6 +654,7 @@ int nuosc_set_mu(nuosc_sim* s, double mu) {
     s->cfg.mu = mu;
     s->ph.mu = mu;
     drop_graph(s);
+    if (s->batch) batch_member_changed(s->batch);
     return NUOSC_OK;
 }
 
@@ -652,6 +663,7 @@ int nuosc_set_pmo(nuosc_sim* s, double pmo) {
     s->cfg.pmo = pmo;
     s->ph.pmo = pmo;
     drop_graph(s);
+    if (s->batch) batch_member_changed(s->batch);
     return NUOSC_OK;
 }
 
@@ -902,12 +914,15 @@ int nuosc_set_zlen(nuosc_sim* s, int zlen) {
     HIPCK(s, hipStreamSynchronize(s->stream));
     s->plan = plan_stage_zlen(s->geo, zlen);
     drop_graph(s);
+    if (s->batch) batch_member_changed(s->batch);
     return NUOSC_OK;
 }
 
 namespace {
 std::mutex g_tune_mu;
-std::map<std::tuple<int, int, int, int>, int> g_tune_cache;
+// key: grid (nx, nz, nv), precision, z boundary, discretisation (KO order,
+// advection scheme: they select the kernel instantiation) and the one-row form
+std::map<std::tuple<int, int, int, int, int, int, int, int>, int> g_tune_cache;
 }  // namespace
 
 
@@ -918,7 +933,7 @@ int nuosc_autotune(nuosc_sim* s, int trials, int* best_zlen, double* best_ms) {
         if (best_ms) *best_ms = -1;
         return NUOSC_OK;
     }
-    const auto key = std::make_tuple(s->nx, s->nz, s->nv, s->cfg.precision);
+    const auto key = std::make_tuple(s->nx, s->nz, s->nv, s->cfg.precision, s->geo.bc, s->geo.ko5, s->geo.adv, s->geo.nox);
     {
         std::lock_guard<std::mutex> g(g_tune_mu);
         auto it = g_tune_cache.find(key);
@@ -973,6 +988,7 @@ int nuosc_autotune(nuosc_sim* s, int trials, int* best_zlen, double* best_ms) {
     const int best = cand[kb];
     s->plan = plan_stage_zlen(s->geo, best);
     drop_graph(s);
+    if (s->batch) batch_member_changed(s->batch);
     {
         std::lock_guard<std::mutex> g(g_tune_mu);
         g_tune_cache[key] = best;
@@ -1172,6 +1188,14 @@ void batch_drop_graph(nuosc_batch* b) {
     b->graph_exec = nullptr;
     b->graph = nullptr;
 }
+void batch_member_changed(nuosc_batch* b) { b->dirty = true; }
+void batch_detach(nuosc_batch* b, nuosc_sim* s) {
+    (void)hipStreamSynchronize(b->stream);
+    b->members.erase(std::remove(b->members.begin(), b->members.end(), s), b->members.end());
+    s->batch = nullptr;
+    b->dirty = true;
+    batch_drop_graph(b);
+}
 }  // namespace
 
 extern "C" {
@@ -1236,6 +1260,7 @@ int nuosc_batch_create(int device, int precision, nuosc_batch** out) {
 int nuosc_batch_destroy(nuosc_batch* b) {
     if (!b) return NUOSC_OK;
     (void)hipStreamSynchronize(b->stream);
+    for (nuosc_sim* m : b->members) m->batch = nullptr;
     batch_drop_graph(b);
     batch_free(b->tables);
     (void)hipStreamDestroy(b->stream);
@@ -1254,9 +1279,10 @@ int nuosc_batch_add(nuosc_batch* b, nuosc_sim* s) {
     if (s->geo.ko5 || s->geo.adv != NUOSC_ADV_CFD4)
         return bfail(b, NUOSC_EINVAL, "only runs with the default discretisation can be batched");
     if (s->nf != kNVar) return bfail(b, NUOSC_EINVAL, "only 2-flavour runs can be batched");
-    for (nuosc_sim* m : b->members)
-        if (m == s) return bfail(b, NUOSC_EINVAL, "already a member");
+    if (s->batch == b) return bfail(b, NUOSC_EINVAL, "already a member");
+    if (s->batch) return bfail(b, NUOSC_EINVAL, "already a member of another batch");
     b->members.push_back(s);
+    s->batch = b;
     b->dirty = true;
     return NUOSC_OK;
 }
@@ -1265,9 +1291,7 @@ int nuosc_batch_remove(nuosc_batch* b, nuosc_sim* s) {
     if (!b || !s) return NUOSC_EINVAL;
     for (size_t k = 0; k < b->members.size(); ++k)
         if (b->members[k] == s) {
-            (void)hipStreamSynchronize(b->stream);
-            b->members.erase(b->members.begin() + k);
-            b->dirty = true;
+            batch_detach(b, s);
             return NUOSC_OK;
         }
     return bfail(b, NUOSC_EINVAL, "not a member");

@@ -316,7 +316,11 @@ __device__ __forceinline__ T flimiter(T a, T b) {
 // Advection term along one axis for the non-default schemes
 // (nuosc_class.cpp:127-178); m[k] / p[k] = the value k cells below / above,
 // f = the scheme's factor (set up per direction), s = sign of the velocity.
-template <typename T>
+// ZAX: the z axis.  The reference's MUSCL form differs between the axes in
+// one sign (nuosc_class.cpp:150-157): for a negative velocity the x term ends
+// in + flimiter(x0 - x-1, x1 - x0) / 2 and the z term in - flimiter(...) / 2;
+// both are reproduced as written.
+template <typename T, bool ZAX>
 __device__ __forceinline__ T adv_axis(int adv, T f, int s, const T (&m)[4], T c0, const T (&p)[4]) {
     switch (adv) {
         case kAdvCfd2:
@@ -333,8 +337,8 @@ __device__ __forceinline__ T adv_axis(int adv, T f, int s, const T (&m)[4], T c0
         case kAdvMuscl: {     // f = -v/h
             const T up = (T)(s > 0) * (c0 + (T)0.5 * flimiter(c0 - m[1], p[1] - c0) - m[1] -
                                        (T)0.5 * flimiter(m[1] - m[2], c0 - m[1]));
-            const T dn = (T)(s < 0) * (p[1] - (T)0.5 * flimiter(p[1] - c0, p[2] - p[1]) - c0 +
-                                       (T)0.5 * flimiter(c0 - m[1], p[1] - c0));
+            const T lc = (T)0.5 * flimiter(c0 - m[1], p[1] - c0);
+            const T dn = (T)(s < 0) * (p[1] - (T)0.5 * flimiter(p[1] - c0, p[2] - p[1]) - c0 + (ZAX ? -lc : lc));
             return f * (up + dn);
         }
         case kAdvOff:
@@ -620,7 +624,7 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& a, int lblock, in
                         zp[kk] = win[f][RAD + kk][e];
                     }
                     xm[0] = xp[0] = zm[0] = zp[0] = c0;
-                    adv = adv_axis<T>(a.g.adv, fz[e], szv, zm, c0, zp) + adv_axis<T>(a.g.adv, fx[e], sxv, xm, c0, xp);
+                    adv = adv_axis<T, true>(a.g.adv, fz[e], szv, zm, c0, zp) + adv_axis<T, false>(a.g.adv, fx[e], sxv, xm, c0, xp);
                     if (a.g.ko5)
                         ko = kz * (zm[3] + zp[3] - (T)6 * (zm[2] + zp[2]) + (T)15 * (zm[1] + zp[1]) - (T)20 * c0) +
                              kx * (xm[3] + xp[3] - (T)6 * (xm[2] + xp[2]) + (T)15 * (xm[1] + xp[1]) - (T)20 * c0);

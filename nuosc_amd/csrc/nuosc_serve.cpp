@@ -4,10 +4,13 @@
 //
 // Requests (one parameter-scan run each) are queued by any host thread.  A
 // dispatcher thread owns the GPU work:
-//   * admission: queued requests are admitted in arrival order while the
-//     summed device footprint of the resident runs stays within the HBM
-//     budget (nuosc_estimate_device_bytes); an admitted run is created,
-//     initialised and analysed at t = 0;
+//   * admission: a queued request is admitted when its device footprint
+//     (nuosc_estimate_device_bytes) fits beside the resident runs' within
+//     the HBM budget.  The queue is scanned in arrival order and any request
+//     that fits is taken, so small runs are not held behind a large one;
+//     once the oldest queued request has waited longer than the age bound,
+//     no younger one is admitted before it (it takes the next memory that
+//     frees).  An admitted run is created, initialised and analysed at t = 0;
 //   * ticks: all resident runs advance together by k steps through one
 //     batch (one launch per RK4 stage for all of them); k stops at the next
 //     analysis or completion step of any run, so every run's series is
@@ -54,6 +57,7 @@ struct nuosc_server {
     int device = 0, precision = NUOSC_PREC_FP64;
     long long budget = 0;
     int max_tick = 10;
+    double age_bound = 30.0;               // seconds a request may be passed over
     std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
 
     std::mutex mu;
@@ -103,13 +107,22 @@ void nuosc_server::retire(Run* r, int status, const std::string& e) {
     cv.notify_all();
 }
 
-// admit queued runs (arrival order) while they fit; called with mu held,
-// drops it around the GPU work of each admission
+// admit queued runs that fit (see the admission rule above); called with mu
+// held, drops it around the GPU work of each admission (submit may append to
+// the queue meanwhile, so each admission rescans it)
 void nuosc_server::admit_locked(std::unique_lock<std::mutex>& lk) {
-    while (!queue.empty()) {
-        Run* r = queue.front();
-        if (resident_bytes + r->bytes > budget && !active.empty()) break;
-        queue.pop_front();
+    while (true) {
+        Run* r = nullptr;
+        const double now = now_s(t0);
+        for (size_t k = 0; k < queue.size(); ++k) {
+            if (resident_bytes + queue[k]->bytes <= budget) {
+                r = queue[k];
+                queue.erase(queue.begin() + (long)k);
+                break;
+            }
+            if (k == 0 && now - queue[0]->t_submit >= age_bound) break;   // the oldest goes first
+        }
+        if (!r) break;
         n_queued = (int)queue.size();
         resident_bytes += r->bytes;
         active.push_back(r);
@@ -220,9 +233,10 @@ int nuosc_server_submit(nuosc_server* s, const nuosc_request* req, long long* id
     nuosc_config c = req->cfg;
     c.precision = s->precision;
     // runs the batch cannot take (nuosc_batch_add) are refused here, not at admission
-    if (c.bc != NUOSC_BC_PERIODIC || c.ko_order == 5 || c.adv != NUOSC_ADV_CFD4 || c.nflavor == 3)
+    if (c.bc != NUOSC_BC_PERIODIC || c.ko_order == 5 || c.adv != NUOSC_ADV_CFD4 || c.nflavor == 3 || c.renorm)
         return NUOSC_EINVAL;
     if (nuosc_estimate_device_bytes(&c, &bytes) != NUOSC_OK) return NUOSC_EINVAL;
+    if (bytes > s->budget) return NUOSC_EINVAL;   // could never be admitted
     Run* r = new Run();
     r->req = *req;
     r->bytes = bytes;
@@ -266,6 +280,14 @@ int nuosc_server_take_result(nuosc_server* s, long long id, double timeout_s, nu
         std::copy(r->series.begin(), r->series.begin() + 5 * n, series);
     }
     delete r;
+    return NUOSC_OK;
+}
+
+int nuosc_server_set_age_bound(nuosc_server* s, double seconds) {
+    if (!s || !(seconds >= 0)) return NUOSC_EINVAL;
+    std::lock_guard<std::mutex> g(s->mu);
+    s->age_bound = seconds;
+    s->cv.notify_all();
     return NUOSC_OK;
 }
 

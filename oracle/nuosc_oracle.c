@@ -524,9 +524,11 @@ static double flimiter(double a, double b) {
 }
 
 /* Advection term of one field at one point along one axis, x = pointer to the
- * point, st = stride along the axis, v = velocity component, h = spacing
- * (nuosc_class.cpp:127-183, non-default branches). */
-static double adv_axis(int adv, const double* x, long st, double v, double h) {
+ * point, st = stride along the axis, v = velocity component, h = spacing,
+ * zax = the z axis (nuosc_class.cpp:127-183, non-default branches).  MUSCL is
+ * written with one sign that differs between the axes (:150-157): for v < 0
+ * the x term ends in + flimiter(x0 - x-1, x1 - x0)/2, the z term in -. */
+static double adv_axis(int adv, const double* x, long st, double v, double h, int zax) {
     switch (adv) {
         case ORC_ADV_CFD2: {
             const double f = -v / (2 * h);
@@ -545,8 +547,9 @@ static double adv_axis(int adv, const double* x, long st, double v, double h) {
         case ORC_ADV_MUSCL: {
             const double up = (v > 0) * (x[0] + 0.5 * flimiter(x[0] - x[-st], x[st] - x[0]) - x[-st] -
                                          0.5 * flimiter(x[-st] - x[-2 * st], x[0] - x[-st]));
+            const double lc = 0.5 * flimiter(x[0] - x[-st], x[st] - x[0]);
             const double dn = (v < 0) * (x[st] - 0.5 * flimiter(x[st] - x[0], x[2 * st] - x[st]) - x[0] +
-                                         0.5 * flimiter(x[0] - x[-st], x[st] - x[0]));
+                                         (zax ? -lc : lc));
             return -v / h * (up + dn);
         }
         case ORC_ADV_OFF:
@@ -573,7 +576,7 @@ static double ko5_axis(const double* x, long st, double eps) {
  * projection of a moment vector onto direction v (INP, :187). */
 #define ADV_FD(x) (dflt ? (factor_z * (((x)[-2 * snv] - (x)[2 * snv]) - 8.0 * ((x)[-snv] - (x)[snv])) + \
                           factor_x * (((x)[-2 * nzv] - (x)[2 * nzv]) - 8.0 * ((x)[-nzv] - (x)[nzv]))) \
-                       : (adv_axis(s->adv, (x), snv, vz[v], s->dz) + adv_axis(s->adv, (x), nzv, vx[v], s->dx)))
+                       : (adv_axis(s->adv, (x), snv, vz[v], s->dz, 1) + adv_axis(s->adv, (x), nzv, vx[v], s->dx, 0)))
 #define KO_FD(x) (s->ko_order != 5 \
                       ? (ko_eps_z * ((x)[-2 * snv] + (x)[2 * snv] - 4 * ((x)[-snv] + (x)[snv]) + 6 * (x)[0]) + \
                          ko_eps_x * ((x)[-2 * nzv] + (x)[2 * nzv] - 4 * ((x)[-nzv] + (x)[nzv]) + 6 * (x)[0])) \

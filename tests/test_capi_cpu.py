@@ -31,7 +31,7 @@ def test_every_declared_symbol_is_exported(nuosc_mod):
 
 def test_identity(nuosc_mod):
     L = nuosc_mod.lib()
-    assert L.nuosc_abi_version() == 4
+    assert L.nuosc_abi_version() == 5
     assert b"gfx950" in L.nuosc_build_info()
 
 

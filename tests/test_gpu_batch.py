@@ -132,3 +132,69 @@ def test_batch_zlen_and_autotune(nuosc_mod, precision):
     b = N.Batch(precision=precision)
     with pytest.raises(N.NuOscError):
         b.add(op)
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_member_physics_change_reaches_batch(nuosc_mod, precision):
+    """set_mu / set_pmo on a member after it joined a batch take effect at the
+    next batch step: the member stays bitwise equal to the same run alone."""
+    N = nuosc_mod
+    alone = [make(N, c, 300 + k, precision) for k, c in enumerate(RUNS[:2])]
+    mixed = [make(N, c, 300 + k, precision) for k, c in enumerate(RUNS[:2])]
+    b = N.Batch(precision=precision)
+    for g in mixed:
+        b.add(g)
+    b.step(3)
+    for g in alone:
+        g.step_rk4(3)
+    for g in (alone[0], mixed[0]):
+        g.set_mu(1.7)
+    for g in (alone[1], mixed[1]):
+        g.set_pmo(-1.0)
+    b.step(4)
+    for g in alone:
+        g.step_rk4(4)
+    for a, m in zip(alone, mixed):
+        assert np.array_equal(a.get_state(), m.get_state())
+    b.close()
+
+
+def test_destroyed_member_leaves_its_batch(nuosc_mod):
+    """Closing a member removes it from its batch (the batch must not keep
+    pointers to freed buffers); the others keep stepping as they would alone.
+    The batch is stepped only after its size shows the member gone."""
+    N = nuosc_mod
+    alone = [make(N, c, 400 + k, 64) for k, c in enumerate(RUNS[:3])]
+    mixed = [make(N, c, 400 + k, 64) for k, c in enumerate(RUNS[:3])]
+    b = N.Batch(precision=64)
+    for g in mixed:
+        b.add(g)
+    b.step(2)
+    for g in alone:
+        g.step_rk4(2)
+    mixed[1].close()
+    assert len(b) == 2
+    b.remove(mixed[1])          # already gone: the binding only forgets it
+    assert len(b.members) == 2
+    b.step(3)
+    for k in (0, 2):
+        alone[k].step_rk4(3)
+        assert np.array_equal(alone[k].get_state(), mixed[k].get_state()), k
+    b.close()
+
+
+def test_member_of_one_batch_only(nuosc_mod):
+    N = nuosc_mod
+    g = make(N, RUNS[0], 1, 64)
+    b1, b2 = N.Batch(precision=64), N.Batch(precision=64)
+    b1.add(g)
+    with pytest.raises(N.NuOscError):
+        b2.add(g)
+    b1.remove(g)
+    b2.add(g)
+    b1.close()
+    b2.close()
+    # a destroyed batch releases its members
+    b3 = N.Batch(precision=64)
+    b3.add(g)
+    g.close()

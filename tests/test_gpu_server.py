@@ -83,3 +83,61 @@ def test_server_respects_budget(nuosc_mod):
     admits = sorted(r[0].t_admit for r in results)
     assert admits[-1] > admits[0]
     srv.close()
+
+
+def test_server_refusals(nuosc_mod):
+    """submit refuses runs a batch cannot take (renormalising, open z, a
+    non-default discretisation, 3 flavours) and runs larger than the budget."""
+    N = nuosc_mod
+    shape = SHAPES[0]
+    one = N.estimate_device_bytes(precision=64, **{k: v for k, v in shape.items() if k != "mu"})
+    srv = N.Server(precision=64, hbm_budget_bytes=int(1.5 * one), max_tick_steps=5)
+    req = N.Request(cfg=N.Config(nn=shape["nv"], gx=2, gz=2, x0=shape["x0"], x1=shape["x1"], z0=shape["z0"],
+                                 z1=shape["z1"], dx=shape["dx"], dz=shape["dz"], cfl=0.4, ko=0.1, mu=1.0,
+                                 renorm=1),
+                    ipt=5, alpha=0.9, eps0=1e-3, sigma=1.0, lnue=0.6, lnueb=0.5, lnuex=N.DBL_MAX,
+                    lnuebx=N.DBL_MAX, seed=1, nsteps=5, ana_every=5)
+    import ctypes as C
+    rid = C.c_longlong()
+    assert N.lib().nuosc_server_submit(srv._h, C.byref(req), C.byref(rid)) == N.NUOSC_EINVAL
+    req.cfg.renorm = 0
+    for field, val in (("bc", N.BC_OPEN_Z), ("ko_order", 5), ("adv", N.ADV_MUSCL), ("nflavor", 3)):
+        setattr(req.cfg, field, val)
+        assert N.lib().nuosc_server_submit(srv._h, C.byref(req), C.byref(rid)) == N.NUOSC_EINVAL, field
+        setattr(req.cfg, field, 0)
+    # twice the grid: larger than the budget on its own
+    req.cfg.x1 = shape["x1"] + (shape["x1"] - shape["x0"])
+    assert N.lib().nuosc_server_submit(srv._h, C.byref(req), C.byref(rid)) == N.NUOSC_EINVAL
+    req.cfg.x1 = shape["x1"]
+    assert N.lib().nuosc_server_submit(srv._h, C.byref(req), C.byref(rid)) == N.NUOSC_OK
+    info, _ = srv.take(timeout=60)
+    assert info.status == 0
+    srv.close()
+
+
+def test_server_admits_small_runs_past_a_large_one(nuosc_mod):
+    """A queued run that does not fit does not hold back younger runs that
+    do (within the age bound); with age bound 0 the order is strict."""
+    N = nuosc_mod
+    small = SHAPES[0]
+    big = dict(SHAPES[0], x0=-1.6, x1=1.6)
+    b_small = N.estimate_device_bytes(precision=64, **{k: v for k, v in small.items() if k != "mu"})
+    b_big = N.estimate_device_bytes(precision=64, **{k: v for k, v in big.items() if k != "mu"})
+    assert b_big > b_small
+    for bound, small_first in ((60.0, True), (0.0, False)):
+        srv = N.Server(precision=64, hbm_budget_bytes=b_big + b_small // 2, max_tick_steps=5)
+        srv.set_age_bound(bound)
+        # a long-running small run occupies memory, so the big one cannot fit;
+        # a second small run arrives after the big one
+        r0 = srv.submit(nsteps=20000, ana_every=20000, seed=1, **small)
+        time.sleep(0.2)
+        r_big = srv.submit(nsteps=5, ana_every=5, seed=2, **big)
+        r_small = srv.submit(nsteps=5, ana_every=5, seed=3, **small)
+        res = {}
+        for _ in range(3):
+            info, _ = srv.take(timeout=120)
+            assert info.status == 0
+            res[info.id] = info
+        srv.close()
+        assert (res[r_small].t_admit < res[r_big].t_admit) == small_first, bound
+        assert res[r_big].t_admit >= res[r0].t_done - 1e-9

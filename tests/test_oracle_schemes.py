@@ -10,8 +10,12 @@ the mathematics fixes rather than by restating their stencils:
     and UPWIND2 2, LOPSIDED and the default 4), exact zero for ADVEC_OFF;
   * upwinding side: the one-sided schemes are dissipative, <f, L f> <= 0 for
     random f in every direction (the downwind choice would be >= 0);
-  * MUSCL with the superbee limiter: exact on linear data, and one value
-    worked by hand from the limiter's definition that minmod would not give.
+  * MUSCL with the superbee limiter: on linear data the x term is exact for
+    either sign of vx, and so is the z term for vz > 0; for vz < 0 the z term
+    as the reference writes it (nuosc_class.cpp:155-157: the last limited
+    slope enters with - where the x term has +) cancels to 0, which the
+    mirrored x form would not give; and one value worked by hand from the
+    limiter's definition that minmod would not give.
 """
 import os
 import sys
@@ -30,10 +34,23 @@ def column(nz, dz, ko_order=3, adv=0, ko=0.0, nn=2, gz=3):
                        pmo=0.0, ko=ko, ko_order=ko_order, adv=adv)
 
 
+def row(nx, dx, adv=0, nn=4):
+    """nx x cells, 4 z cells, mu = 0, no vacuum term, no KO: with data
+    constant in z the RHS is the x advection term alone."""
+    return O.OracleSim(nn=nn, gx=3, gz=3, x0=0.0, x1=nx * dx, z0=0.0, z1=0.4, dx=dx, dz=0.1, mu=0.0,
+                       pmo=0.0, ko=0.0, adv=adv)
+
+
 def profile_state(o, prof):
     """Every field and direction carries the z profile prof[j]."""
     nx, nz, nv = o.shape
     return np.broadcast_to(prof[None, None, :, None], (8, nx, nz, nv)).copy()
+
+
+def profile_state_x(o, prof):
+    """Every field and direction carries the x profile prof[i]."""
+    nx, nz, nv = o.shape
+    return np.broadcast_to(prof[None, :, None, None], (8, nx, nz, nv)).copy()
 
 
 @pytest.mark.parametrize("p", range(7))
@@ -95,27 +112,46 @@ def test_advection_off_is_zero():
 @pytest.mark.parametrize("adv", [O.OracleSim.ADV_UPWIND2, O.OracleSim.ADV_LOPSIDED, O.OracleSim.ADV_MUSCL])
 def test_one_sided_schemes_are_upwind(adv):
     """<f, L f> <= 0 per direction: the scheme takes its extra points on the
-    side the flow comes from."""
-    nz, dz = 40, 0.1
-    o = column(nz, dz, adv=adv, nn=4)
+    side the flow comes from.  Along x for every moving direction; along z
+    for every moving direction, except MUSCL's vz < 0 branch, whose z form as
+    written is not a flux difference (test_muscl_linear_data)."""
+    n, h = 40, 0.1
     rng = np.random.default_rng(8)
-    f = rng.standard_normal(nz)
+    f = rng.standard_normal(n)
+    o = column(n, h, adv=adv, nn=4)
     r = o.calrhs(profile_state(o, f))[0, 0]    # (nz, nv)
     vz = o.vgrid()[2]
     energy = np.einsum("j,jv->v", f, r)
-    moving = np.abs(vz) > 1e-12
+    moving = (vz > 1e-12) if adv == O.OracleSim.ADV_MUSCL else (np.abs(vz) > 1e-12)
+    assert np.all(energy[moving] < 0), energy[moving].max()
+    o.close()
+    o = row(n, h, adv=adv)
+    r = o.calrhs(profile_state_x(o, f))[0, :, 0]   # (nx, nv)
+    vx = o.vgrid()[0]
+    energy = np.einsum("i,iv->v", f, r)
+    moving = np.abs(vx) > 1e-12
     assert np.all(energy[moving] < 0), energy[moving].max()
     o.close()
 
 
-def test_muscl_exact_on_linear_data():
-    nz, dz = 30, 0.1
-    o = column(nz, dz, adv=O.OracleSim.ADV_MUSCL, nn=4)
-    slope = 0.7
-    r = o.calrhs(profile_state(o, slope * np.arange(nz) * dz))[0, 0]
+def test_muscl_linear_data():
+    """Slope s: the x term is -vx s for either sign of vx (the limited slopes
+    of a linear profile are s); the z term is -vz s for vz > 0, and for vz < 0
+    it is 0 as the reference writes it: x1 - phi(s, s)/2 - x0 - phi(s, s)/2
+    = s - s/2 - s/2 (the x form has + phi/2 there and gives s)."""
+    n, h, slope = 30, 0.1, 0.7
+    o = column(n, h, adv=O.OracleSim.ADV_MUSCL, nn=4)
+    r = o.calrhs(profile_state(o, slope * np.arange(n) * h))[0, 0]
     vz = o.vgrid()[2]
-    inner = r[3:nz - 3]
-    assert np.allclose(inner, -vz[None, :] * slope, rtol=1e-12, atol=1e-12)
+    want = np.where(vz > 0, -vz * slope, 0.0)
+    assert np.allclose(r[3:n - 3], want[None, :], rtol=1e-12, atol=1e-12)
+    assert np.any(vz < -0.5)   # the vz < 0 branch is exercised
+    o.close()
+    o = row(n, h, adv=O.OracleSim.ADV_MUSCL)
+    r = o.calrhs(profile_state_x(o, slope * np.arange(n) * h))[0, :, 0]
+    vx = o.vgrid()[0]
+    assert np.allclose(r[3:n - 3], -vx[None, :] * slope, rtol=1e-12, atol=1e-12)
+    assert np.any(vx < -0.5)
     o.close()
 
 
