@@ -120,3 +120,53 @@ def test_cli_open_boundary_and_growth_file(tmp_path, oracle_mod):
         t, r = map(float, grow[k].split())
         assert t == float(exp[0]) or abs(t - float(exp[0])) <= 1e-12
         assert abs(r - rex) <= 1e-9 * rex
+
+
+def ostream_g(x):
+    """A double as std::ostream writes it with the default flags and
+    precision 6: the %g conversion (C and Python agree on it)."""
+    return "%g" % x
+
+
+def test_cli_p3_snapshot_parity(tmp_path, oracle_mod):
+    """P3 snapshots against the oracle and the reference's writer
+    (nuosc_snapshot.cpp:13-41, 82-122):
+      * .meta byte for byte: 'dt sx nz sv', 'x0 x1', 'z0 z1', then Z[0..nz),
+        X[xidx], vx[vidx], vy[vidx], vz[vidx], each value followed by a blank,
+        every number as std::ostream writes it at precision 6;
+      * .bin: int32 step, f64 phy_time (bitwise the oracle's accumulated
+        time), then f64[sx][nz][sv] = (ee - mm)/G0 at x = 0, nx/2, nx-1 and
+        the directions ((nn-1)/2) nn + v, v < nn (nuosc.cpp:169-175), within
+        1e-10 of P3 from the oracle's state after the same steps."""
+    p = run_cli(tmp_path, ARGS + ["--DUMP_EVERY", "15", "--snap_p3", "1"])
+    assert p.returncode == 0, p.stderr
+    o = oracle_mod.OracleSim(nn=3, x0=-0.8, x1=0.8, z0=-1.6, z1=1.6, dx=0.1, dz=0.1, cfl=0.4, ko=0.1, mu=1.0,
+                             pmo=1.0)
+    o.fill_init(ipt=0, alpha=0.9, eps0=1e-3, sigma=1.0, lnue=0.6, lnueb=0.5)
+    nx, nz, nv = o.shape
+    xidx = [0, nx // 2, nx - 1]
+    vidx = [((3 - 1) // 2) * 3 + v for v in range(3)]
+    vx, vy, vz, _ = o.vgrid()
+    X = -0.8 + (np.arange(nx) + 0.5) * 0.1
+    Z = -1.6 + (np.arange(nz) + 0.5) * 0.1
+    lines = [" ".join(ostream_g(v) for v in (o.s.dt,)) + " " + " ".join(str(n) for n in (len(xidx), nz, len(vidx))),
+             f"{ostream_g(-0.8)} {ostream_g(0.8)}", f"{ostream_g(-1.6)} {ostream_g(1.6)}",
+             "".join(ostream_g(z) + " " for z in Z), "".join(ostream_g(X[i]) + " " for i in xidx),
+             "".join(ostream_g(vx[v]) + " " for v in vidx), "".join(ostream_g(vy[v]) + " " for v in vidx),
+             "".join(ostream_g(vz[v]) + " " for v in vidx)]
+    want_meta = "".join(line + "\n" for line in lines).encode()
+    assert open(tmp_path / "P3_000000.bin.meta", "rb").read() == want_meta
+    G0 = o.G0
+    for t in (0, 15, 30):
+        if t:
+            o.step(15)
+        raw = open(tmp_path / f"P3_{t:06d}.bin", "rb").read()
+        assert len(raw) == 4 + 8 + 8 * len(xidx) * nz * len(vidx)
+        step, = struct.unpack_from("<i", raw, 0)
+        ptime, = struct.unpack_from("<d", raw, 4)
+        assert step == t and ptime == o.phy_time
+        got = np.frombuffer(raw[12:], dtype="<f8").reshape(len(xidx), nz, len(vidx))
+        st = o.get_state()
+        p3 = (st[0] - st[1]) / G0[None, None, :]
+        want = p3[xidx][:, :, vidx]
+        assert np.max(np.abs(got - want)) <= 1e-10 * np.max(np.abs(want)), t
