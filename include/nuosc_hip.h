@@ -225,6 +225,15 @@ int nuosc_state_updated(nuosc_sim* sim);
  * in eval_conserved, nuosc_ana.cpp:14): dst[sx][nz][sv] with
  * P3 = (ee-mm)/G0 at x indices xidx[sx] and direction indices vidx[sv]. */
 int nuosc_p3_slice(nuosc_sim* sim, const int* xidx, int sx, const int* vidx, int sv, double* dst);
+/* No counterpart in the reference.  The same slice for callers that cannot
+ * allocate device memory (serving loops, code between graph replays,
+ * harnesses that forbid allocation): nuosc_p3_slice may hipMalloc a larger
+ * staging buffer; this variant stages the index lists and the output in a
+ * fixed 16 MiB device array that each GPU holds for the whole process, and
+ * calls no allocator.  Limits: sx + sv <= 16384 and sx * nz * sv <= 2088960,
+ * else NUOSC_EINVAL.  Calls on one GPU share the array: calls of this
+ * process serialise on a per-device lock (each call is synchronous). */
+int nuosc_p3_slice_noalloc(nuosc_sim* sim, const int* xidx, int sx, const int* vidx, int sv, double* dst);
 
 /* Wall-clock of the device work of the last nuosc_step_rk4 call (ms),
  * from HIP events on the handle's stream (synchronises). */

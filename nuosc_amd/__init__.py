@@ -125,6 +125,7 @@ SIGNATURES = {
     "nuosc_upload_field": ([_vp, C.c_int, _dp], C.c_int),
     "nuosc_state_updated": ([_vp], C.c_int),
     "nuosc_p3_slice": ([_vp, _ip, C.c_int, _ip, C.c_int, _dp], C.c_int),
+    "nuosc_p3_slice_noalloc": ([_vp, _ip, C.c_int, _ip, C.c_int, _dp], C.c_int),
     "nuosc_last_step_ms": ([_vp, _dp], C.c_int),
     "nuosc_profile_stages": ([_vp, C.c_int, _dp], C.c_int),
     "nuosc_get_plan": ([_vp, _ip, _ip], C.c_int),
@@ -373,12 +374,15 @@ class NuOsc:
             self._check(lib().nuosc_upload_field(self._h, f, _ptr(np.ascontiguousarray(a[f]).reshape(-1))))
         self._check(lib().nuosc_state_updated(self._h))
 
-    def p3_slice(self, xidx, vidx):
+    def p3_slice(self, xidx, vidx, noalloc=False):
+        """P3 at x indices xidx and direction indices vidx: (sx, nz, sv).
+        noalloc: through nuosc_p3_slice_noalloc (no device allocation; the
+        slice size is bounded)."""
         xi = np.ascontiguousarray(xidx, dtype=np.int32)
         vi = np.ascontiguousarray(vidx, dtype=np.int32)
         out = np.zeros(len(xi) * self.nz * len(vi))
-        self._check(lib().nuosc_p3_slice(self._h, xi.ctypes.data_as(_ip), len(xi),
-                                         vi.ctypes.data_as(_ip), len(vi), _ptr(out)))
+        fn = lib().nuosc_p3_slice_noalloc if noalloc else lib().nuosc_p3_slice
+        self._check(fn(self._h, xi.ctypes.data_as(_ip), len(xi), vi.ctypes.data_as(_ip), len(vi), _ptr(out)))
         return out.reshape(len(xi), self.nz, len(vi))
 
 

@@ -36,7 +36,7 @@ HOST_FLAGS = [
     "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include",
 ]
 
-KERNEL_SRCS = ["nuosc_kernels.hip", "nuosc_flavor3.hip"]
+KERNEL_SRCS = ["nuosc_kernels.hip", "nuosc_flavor3.hip", "nuosc_noalloc.hip"]
 HOST_SRCS = ["nuosc_capi.cpp", "vgrid.cpp", "nuosc_serve.cpp"]
 CLI_SRCS = ["NuOsc.cpp", "nuosc_main.cpp"]
 

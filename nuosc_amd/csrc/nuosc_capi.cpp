@@ -436,6 +436,13 @@ hipError_t enqueue_scratch_stages(nuosc_sim* s) {
 }
 }  // namespace
 
+namespace nuosc {
+SimView sim_view(const nuosc_sim* s) {
+    return SimView{s->geo, s->st_u, s->d_g0, (int)s->esize, s->cfg.device, s->stream};
+}
+int sim_fail(nuosc_sim* s, int code, const char* msg) { return fail(s, code, msg); }
+}  // namespace nuosc
+
 extern "C" {
 
 int nuosc_abi_version(void) { return NUOSC_ABI_VERSION; }

@@ -34,6 +34,8 @@
 #include <cstdint>
 #include <vector>
 
+struct nuosc_sim;   // the handle (nuosc_capi.cpp)
+
 namespace nuosc {
 
 constexpr int kNVar = 8;         // fields of the 2-flavour state
@@ -258,6 +260,20 @@ hipError_t launch_stage3(int stage, const Geometry& g, const Physics& ph, const 
                          T* yout, const double* mom, double* part, double cw, double cy, hipStream_t s,
                          double ca = 0.0, double cb = 0.0);
 hipError_t launch_moment_totals3(const Geometry& g, const double* part, double* mom, hipStream_t s);
+
+// What an entry point implemented beside the kernels needs of a handle: its
+// geometry, current state u, G0 | G0b, value size, device and stream.
+struct SimView {
+    Geometry geo;
+    const void* state;
+    const double* g0;
+    int esize;
+    int device;
+    hipStream_t stream;
+};
+SimView sim_view(const nuosc_sim* s);
+// records msg as the handle's last error and returns code
+int sim_fail(nuosc_sim* s, int code, const char* msg);
 
 // Renormalisation of the state to |P| = 1 (the reference's formula).
 template <typename T>
