@@ -42,18 +42,22 @@
 namespace nuosc {
 
 // Stage-kernel wave shape per precision: R x rows of 64/R lanes, V adjacent
-// directions per lane (a row segment of one field is one 128-byte line).
-// FP64 4 x 16 lanes x 1; FP32 2 x 32 lanes x 1 (measured faster than
-// 4 x 16 lanes x 2: it keeps 3 waves per SIMD).
+// directions per lane.  FP64 1 x 64 lanes x 1 (a row segment of one field is
+// four 128-byte lines); FP32 2 x 32 lanes x 1 (one line).  Wider direction
+// chunks mean fewer partial moment records per site (11 instead of 41 at 642
+// directions in FP64) and fewer reads of each site's moment totals, for more
+// block-edge rows (read mostly from L2): at 512 x 512 x 642 the FP64 step is
+// 6% faster with 1 x 64 lanes than with 4 x 16 (2 x 32: 3.5%); FP32 gains
+// nothing from 1 x 64.
 template <typename T>
 struct StageShape;
 // W waves per block: a block covers W R adjacent x rows of one chunk and
-// loads the 4 rows beyond its edges as halos.  4 waves in both precisions:
-// FP32 blocks of 8 waves (16 rows) read fewer edge rows but measured slower
-// (the per-step barrier waits for more waves).
+// loads the rows beyond its edges as halos.  4 waves in both precisions:
+// blocks of 8 waves read fewer edge rows but measured slower (the per-step
+// barrier waits for more waves).
 template <>
 struct StageShape<double> {
-    static constexpr int R = 4, V = 1, W = 4;
+    static constexpr int R = 1, V = 1, W = 4;
 };
 template <>
 struct StageShape<float> {
@@ -433,16 +437,25 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& a, int lblock, in
     // stage 7 rewrites u in place, so a duplicate store could reach u before
     // the owner has read it)
     const unsigned vos = (unsigned)(row_ok && (!NOX || ch < nch) ? own + kXg : spare) * xrowb + dof;
-    // Edge rows: edge e = wib + 4g (row group g < 2 of wave wib, e < 2 RAD)
+    // Edge rows: edge e = wib + W g (row group g < LG of wave wib, e < 2 RAD)
     // loads block row e - RAD (e < RAD) or WR + e - RAD with its L lanes.  The
     // other lanes' offset lies past the descriptor's range, so they fetch
-    // nothing (the load returns zeros), and they write the idle slot.
-    const int eidx = wib + 4 * g;
-    const bool eln = g < 2 && eidx < 2 * RAD;
-    const int eb = eidx < RAD ? eidx - RAD : W * R + eidx - RAD;   // block row
-    const int erow = min(eb + kXg, spare);                           // stored row relative to ib
-    const unsigned vh = eln ? (unsigned)erow * xrowb + dof : wbytes;
-    const int slot_h = (eln ? eb + RAD : W * R + 2 * RAD) * D + dl * V;
+    // nothing (the load returns zeros), and they write the idle slot.  With
+    // one row group per wave and more edge rows than waves (radius 3, one x
+    // row per wave), the loaders of edges W LG .. 2 RAD - 1 load a second one.
+    constexpr int NE = 2 * RAD;
+    constexpr int LG = R >= 2 ? 2 : 1;
+    constexpr bool E2 = NE > W * LG;
+    auto edge_row = [&](int e, bool ok, unsigned& voff, int& slot) {
+        const int eb = e < RAD ? e - RAD : W * R + e - RAD;   // block row
+        const int erow = min(eb + kXg, spare);                  // stored row relative to ib
+        voff = ok ? (unsigned)erow * xrowb + dof : wbytes;
+        slot = (ok ? eb + RAD : W * R + 2 * RAD) * D + dl * V;
+    };
+    unsigned vh, vh2;
+    int slot_h, slot_h2;
+    edge_row(wib + W * g, g < LG && wib + W * g < NE, vh, slot_h);
+    edge_row(W * LG + wib + W * g, E2 && g < LG && W * LG + wib + W * g < NE, vh2, slot_h2);
     const int slot_c = (own + RAD) * D + dl * V;
 
     // per-direction constants (nuosc_class.cpp:119-121, 180-181)
@@ -499,6 +512,7 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& a, int lblock, in
 
     T win[kNVar][NZW][V]; // own row, z = j-RAD .. j+RAD
     T hreg[kNVar][V];     // edge row of z = j+2, staged into LDS in step j+1
+    T hreg2[E2 ? kNVar : 1][V];   // the second edge row (E2)
     // velocity moments of the stage input at this lane's site (one value per
     // lane) for z = j, loaded at the end of step j-1
     unsigned long long bm = a.mtot;
@@ -527,6 +541,17 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& a, int lblock, in
         }
 #pragma unroll
         for (int f = 0; f < kNVar; ++f) yin.ldv(yin.base, f, vh, s1, hreg[f]);
+        if constexpr (E2) {
+#pragma unroll
+            for (int f = 0; f < kNVar; ++f) {
+                T x[V];
+                yin.ldv(yin.base, f, vh2, so, x);
+#pragma unroll
+                for (int e = 0; e < V; ++e) xs[j0 & 1][f][slot_h2 + e] = x[e];
+            }
+#pragma unroll
+            for (int f = 0; f < kNVar; ++f) yin.ldv(yin.base, f, vh2, s1, hreg2[f]);
+        }
     }
     double Ml = load_moment(j0);
 
@@ -547,6 +572,12 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& a, int lblock, in
             for (int f = 0; f < kNVar; ++f)
 #pragma unroll
                 for (int e = 0; e < V; ++e) xs[lv ^ 1][f][slot_h + e] = hreg[f][e];
+            if constexpr (E2) {
+#pragma unroll
+                for (int f = 0; f < kNVar; ++f)
+#pragma unroll
+                    for (int e = 0; e < V; ++e) xs[lv ^ 1][f][slot_h2 + e] = hreg2[f][e];
+            }
         }
         // ---- the step's loads, in the order they are consumed: own row
         // z+RAD (stencil), u and w (update), edge row z+2 (staged in the next
@@ -574,6 +605,10 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& a, int lblock, in
         if constexpr (!NOX) {
 #pragma unroll
             for (int f = 0; f < kNVar; ++f) yin.ldv(by, f, vh, s2, hreg[f]);
+            if constexpr (E2) {
+#pragma unroll
+                for (int f = 0; f < kNVar; ++f) yin.ldv(by, f, vh2, s2, hreg2[f]);
+            }
             __builtin_amdgcn_sched_barrier(0);
         }
 
