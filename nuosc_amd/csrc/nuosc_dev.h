@@ -59,7 +59,7 @@ enum AnaIdx {
 
 struct Geometry {
     int nx, nz, nv, nvp;
-    int lpr;               // directions per chunk: one 128-byte row segment (dirs_per_chunk)
+    int lpr;               // directions per chunk (dirs_per_chunk)
     int rows;              // x rows per stage-kernel wavefront (rows_per_wave)
     int wpb;               // wavefronts per stage-kernel block (waves_per_block)
     int nch;               // direction chunks per site = ceil(nv / lpr)
@@ -141,10 +141,10 @@ struct StagePlan {
 // advection, else 2
 inline int scheme_radius(int ko5, int adv) { return (ko5 || adv == kAdvLopsided) ? 3 : 2; }
 
-// x-rows per wavefront of the stage kernel (FP64 4 rows x 16 directions,
-// FP32 2 rows x 32 directions: one 128-byte row segment per field and row)
+// x-rows per wavefront of the stage kernel (1 row x 64 directions in both
+// precisions)
 int rows_per_wave(int esize);
-// directions per chunk = one 128-byte row segment (FP64 16, FP32 32)
+// directions per chunk (64)
 int dirs_per_chunk(int esize);
 // wavefronts per stage-kernel block (4)
 int waves_per_block(int esize);

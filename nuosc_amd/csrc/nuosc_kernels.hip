@@ -1,7 +1,7 @@
 // nuosc_kernels.hip -- CDNA4 (gfx950) kernels of the nuosc RK4 time stepper.
 //
 // The hot path is k_stage: one launch per RK4 stage.  Each wavefront owns R
-// adjacent x-rows, one 128-byte direction chunk of v and one z-segment, and
+// adjacent x-rows, one chunk of 64/R directions and one z-segment, and
 // marches along z keeping a 5-point z-window of all 8 fields in registers
 // (every value of its rows is loaded once); the x-neighbours come from a
 // small LDS slab filled with the wave's own rows and 4 halo rows, which the
@@ -42,13 +42,14 @@
 namespace nuosc {
 
 // Stage-kernel wave shape per precision: R x rows of 64/R lanes, V adjacent
-// directions per lane.  FP64 1 x 64 lanes x 1 (a row segment of one field is
-// four 128-byte lines); FP32 2 x 32 lanes x 1 (one line).  Wider direction
-// chunks mean fewer partial moment records per site (11 instead of 41 at 642
-// directions in FP64) and fewer reads of each site's moment totals, for more
-// block-edge rows (read mostly from L2): at 512 x 512 x 642 the FP64 step is
-// 6% faster with 1 x 64 lanes than with 4 x 16 (2 x 32: 3.5%); FP32 gains
-// nothing from 1 x 64.
+// directions per lane.  Both precisions 1 x 64 lanes x 1: a row segment of
+// one field is four 128-byte lines in FP64, two in FP32.  Wide direction
+// chunks mean fewer partial moment records per site (11 at 642 directions,
+// instead of 41 with 4 x 16 lanes in FP64 or 21 with 2 x 32 in FP32) and
+// fewer reads of each site's moment totals, for more block-edge rows (read
+// mostly from L2): at 512 x 512 x 642 the FP64 step is 6% faster with 1 x 64
+// lanes than with 4 x 16 (2 x 32: 3.5%), the FP32 step 2% faster than with
+// 2 x 32.  Two directions per lane (FP32) need ~250 VGPRs and ran 7% slower.
 template <typename T>
 struct StageShape;
 // W waves per block: a block covers W R adjacent x rows of one chunk and
@@ -61,7 +62,7 @@ struct StageShape<double> {
 };
 template <>
 struct StageShape<float> {
-    static constexpr int R = 2, V = 1, W = 4;
+    static constexpr int R = 1, V = 1, W = 4;
 };
 template <typename T>
 constexpr int kVals = StageShape<T>::V;
@@ -218,9 +219,9 @@ typedef const __attribute__((address_space(4))) double cdouble;
 // ---------------------------------------------------------------------------
 // RK4 stage kernel
 //
-// A wavefront covers R adjacent x-rows x L = 64/R directions (FP64: 4 x 16,
-// FP32: 2 x 32, so every row segment is one 128-byte line) and marches along
-// z over one segment.  Per lane it keeps a 5-deep z-window of its own row for
+// A wavefront covers R adjacent x-rows x L = 64/R directions (R = 1 in both
+// precisions: a row segment of one field is 512 bytes in FP64, 256 in FP32)
+// and marches along z over one segment.  Per lane it keeps a 5-deep z-window of its own row for
 // all 8 fields in registers.  The four waves of a block cover 4R adjacent
 // rows of one direction chunk and move along z together (one LDS barrier
 // per step): each writes its own rows into a block-shared two-level LDS slab
@@ -376,7 +377,7 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& a, int lblock, in
     constexpr int NZW = 2 * RAD + 1;   // z window
     constexpr int V = kVals<T>;    // directions per lane (one 8-byte load)
     constexpr int L = kWave / R;   // lanes per row
-    constexpr int D = L * V;       // directions per row chunk (one 128-byte line)
+    constexpr int D = L * V;       // directions per row chunk
     auto& xs = slab.v;
 
     const int lane = threadIdx.x & 63;
@@ -728,8 +729,16 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& a, int lblock, in
             }
             // lane dl of a row holds the row total of value dl >> sh (values
             // 12..15 are zeros, stored into the spare row's records, which
-            // nothing reads; lanes sharing a value store the same bits)
-            const double tot = group_sum16<L, COMP>(t, te, lane);
+            // nothing reads; lanes sharing a value store the same bits).
+            // The sum over the row's lanes is an FP64 pairwise tree (log2 L
+            // levels: its error is at most log2(L) 2^-53 of the summed
+            // magnitudes), compensated only where a lane carries several
+            // directions' rounding errors; the chunk records of a site are
+            // added with compensation (k_moment_totals) in the FP32 form.
+            // (A compensated tree measured 13% slower in FP32 stage 1 and
+            // changes the moments by ~1e-16 relative, far below FP32
+            // rounding of the state.)
+            const double tot = group_sum16<L, COMP && (V > 1)>(t, te, lane);
             constexpr int sh = (L == 16) ? 0 : (L == 32 ? 1 : 2);
             unsigned long long bp = a.mpart;
             asm volatile("" : "+s"(bp));
