@@ -9,8 +9,9 @@ nuosc_class.cpp:28-82, 189-277, 543-569) against the oracle:
   * an open-z column, 20 RK4 steps: relative L-inf <= 1e-10 per field;
   * calRHS of the FP32 form against FP64 on a state exactly representable in
     FP32: within 64 eps32 max|field| / h (FP32 rounding of the stencil);
-  * 50 RK4 steps of the FP32 form against FP64: Pee within 1e-5, fields
-    within 1e-4 of max|field|;
+  * 50 RK4 steps of the FP32 form against FP64: Pee and fields within
+    4 * 50 * 2^-24 * 2 = 2.4e-5 (of max|field|): one FP32 rounding per stage
+    store, never cancelling, amplified at most twofold;
   * renormalising runs, 10 steps: relative L-inf <= 1e-10 per field;
   * hipGraph replay bitwise equal to eager launches; per-stage timing;
   * the CLI with --nflavor 3, and the arguments the path refuses.
@@ -249,10 +250,13 @@ def test_server_refuses_flavor3(nuosc_mod):
 
 def test_flavor3_fp32_tracks_fp64(nuosc_mod):
     """50 RK4 steps of the FP32 form against FP64 from the same seeded
-    state: Pee within 1e-5, fields within 1e-4 of max|field| (FP32 rounding
-    accumulated over the run; measured 1.7e-6 and 2.8e-6, growing about
-    linearly with the step count).  The 3-flavour vacuum terms as written do
-    not conserve Tr rho, so Pee drifts above 1 here in both precisions."""
+    state.  Bound: every stage store rounds each field to FP32 (relative
+    u = 2^-24), 4 stores per step; if those roundings never cancel and the
+    dynamics amplify an error at most twofold over the run (t = 2), the
+    FP32 state is within 4 * 50 * u * 2 = 2.4e-5 of max|field|, and so is
+    Pee = sum w ee / sum w G0, a weighted mean of ee/G0.  (Measured: fields
+    2.8e-6, Pee 1.7e-6.)  The 3-flavour vacuum terms as written do not
+    conserve Tr rho, so Pee drifts above 1 here in both precisions."""
     N = nuosc_mod
     c = cfg_for(12, 16, 4, ko=0.05, pmo=0.5)
     init = dict(ipt=5, alpha=0.9, eps0=1e-3, sigma=1.0, lnue=0.6, lnueb=0.5, seed=8)
@@ -262,8 +266,9 @@ def test_flavor3_fp32_tracks_fp64(nuosc_mod):
         g.fillInitValue(**init)
         g.step_rk4(50)
     sa, sb = a.get_state(), b.get_state()
-    assert np.max(np.abs(sa - sb)) <= 1e-4 * np.max(np.abs(sa))
+    bound = 4 * 50 * 2.0 ** -24 * 2
+    assert np.max(np.abs(sa - sb)) <= bound * np.max(np.abs(sa))
     assert np.any(sa[8:])                        # the vacuum term populated the tau fields
-    assert abs(a.analysis().surv - b.analysis().surv) <= 1e-5
+    assert abs(a.analysis().surv - b.analysis().surv) <= bound
     a.close()
     b.close()
