@@ -6,7 +6,8 @@ nuosc_class.cpp:28-82, 189-277, 543-569) against the oracle:
   * 30 RK4 steps, FP64, from a state with all 18 fields nonzero:
     relative L-inf <= 1e-10 per field; the analysis line agrees to 1e-12;
   * 12 SSP-RK3 steps, FP64: relative L-inf <= 1e-10 per field;
-  * an open-z column, 20 RK4 steps: relative L-inf <= 1e-10 per field;
+  * open z (one x row and a 6-row box), 20 RK4 steps: relative L-inf <=
+    1e-10 per field;
   * calRHS of the FP32 form against FP64 on a state exactly representable in
     FP32: within 64 eps32 max|field| / h (FP32 rounding of the stencil);
   * 50 RK4 steps of the FP32 form against FP64: Pee and fields within
@@ -114,10 +115,11 @@ def test_srk3_flavor3(oracle_mod, nuosc_mod):
     o.close()
 
 
-def test_open_column_flavor3(oracle_mod, nuosc_mod):
+@pytest.mark.parametrize("x1", [0.1, 0.6])   # one x row (nx = 1) and a 6-row box (nx = 6)
+def test_open_column_flavor3(oracle_mod, nuosc_mod, x1):
     O, N = oracle_mod, nuosc_mod
-    c = dict(nv=2, x0=0.0, x1=0.1, z0=-3.2, z1=3.2, dx=0.1, dz=0.1, CFL=0.4, ko=0.05, mu=1.0, pmo=0.3)
-    o = O.OracleSim(nn=2, gx=2, gz=2, x0=0.0, x1=0.1, z0=-3.2, z1=3.2, dx=0.1, dz=0.1, cfl=0.4, ko=0.05, mu=1.0,
+    c = dict(nv=2, x0=0.0, x1=x1, z0=-3.2, z1=3.2, dx=0.1, dz=0.1, CFL=0.4, ko=0.05, mu=1.0, pmo=0.3)
+    o = O.OracleSim(nn=2, gx=2, gz=2, x0=0.0, x1=x1, z0=-3.2, z1=3.2, dx=0.1, dz=0.1, cfl=0.4, ko=0.05, mu=1.0,
                     pmo=0.3, bc=1, zc=-1.0, nflavor=3)
     g = N.NuOsc(**c, bc=N.BC_OPEN_Z, zc=-1.0, nflavor=3)
     init = dict(ipt=0, alpha=0.9, eps0=1e-2, sigma=0.5, lnue=0.6, lnueb=0.5)
