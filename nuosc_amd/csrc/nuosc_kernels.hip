@@ -769,18 +769,19 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& a, int lblock, in
     }
 }
 
-// Waves per SIMD the register allocation aims at: FP64 stages 0-5 fit 168
-// VGPRs (3 waves), the SSP-RK3 stages 6 and 7 would spill there (2 waves);
-// the FP32 stages fit 128 (4 waves).
-template <typename T, int STAGE, int RAD = 2>
+// Waves per SIMD the register allocation aims at: the FP64 slab kernels are
+// held to 2 by their LDS (two 73.7 KB blocks of 4 waves per CU); the FP64
+// one-row (NOX) stages 0-5 fit 168 VGPRs (3 waves), the SSP-RK3 stages 6 and
+// 7 would spill there (2 waves); the FP32 stages fit 128 (4 waves).
+template <typename T, int STAGE, int RAD = 2, bool NOX = false>
 constexpr int stage_waves() {
-    return RAD == 3 ? (sizeof(T) == 4 ? 2 : 1) : (sizeof(T) == 4 ? 4 : (STAGE <= 5 ? 3 : 2));
+    return RAD == 3 ? (sizeof(T) == 4 ? 2 : 1) : (sizeof(T) == 4 ? 4 : (STAGE <= 5 && NOX ? 3 : 2));
 }
 
 // RAD = 2: the reference's default discretisation; RAD = 3: any of the
 // schemes, chosen at run time (Geometry::ko5, Geometry::adv)
 template <typename T, int R, int STAGE, bool COMP, int RAD = 2, bool NOX = false>
-__global__ __launch_bounds__(64 * StageShape<T>::W) __attribute__((amdgpu_waves_per_eu(stage_waves<T, STAGE, RAD>()))) void k_stage(const StageArgs<T> a) {
+__global__ __launch_bounds__(64 * StageShape<T>::W) __attribute__((amdgpu_waves_per_eu(stage_waves<T, STAGE, RAD, NOX>()))) void k_stage(const StageArgs<T> a) {
     constexpr int W = StageShape<T>::W;
     __shared__ Slab<T, R, W, RAD> slab;   // (never touched by NOX, which the compiler then leaves unallocated)
     const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
