@@ -31,6 +31,7 @@
 #include <hip/hip_runtime.h>
 
 #include "nuosc_dev.h"
+#include "nuosc_lanes.h"
 
 #include <algorithm>
 
@@ -43,29 +44,6 @@ constexpr int kF3Tile = 16;   // z sites per tile of the logical block order
 enum F3Field { EE, MM, EMR, EMI, BEE, BMM, BEMR, BEMI, TT, MTR, MTI, TER, TEI, BTT, BMTR, BMTI, BTER, BTEI };
 
 typedef const __attribute__((address_space(4))) double cdouble3;
-
-// Keeps half of M values and adds the partner's other half (lane ^ d): the
-// lanes with bit d set keep the upper half.
-template <int M>
-__device__ __forceinline__ void halve_xor(const double* in, double* out, int d, bool hi) {
-#pragma unroll
-    for (int q = 0; q < M / 2; ++q) {
-        const double keep = hi ? in[q + M / 2] : in[q];
-        const double send = hi ? in[q] : in[q + M / 2];
-        out[q] = keep + __shfl_xor(send, d);
-    }
-}
-
-// Wave total of 32 per-lane values: lane L ends with the total of value L >> 1.
-__device__ __forceinline__ double wave_sum32(const double (&a)[32], int lane) {
-    double b[16], c[8], d[4], e[2], f[1];
-    halve_xor<32>(a, b, 32, lane & 32);
-    halve_xor<16>(b, c, 16, lane & 16);
-    halve_xor<8>(c, d, 8, lane & 8);
-    halve_xor<4>(d, e, 4, lane & 4);
-    halve_xor<2>(e, f, 2, lane & 2);
-    return f[0] + __shfl_xor(f[0], 1);
-}
 
 // nuosc_class.cpp:57-71: moment differences of one point
 template <typename T>
@@ -611,13 +589,7 @@ __device__ __forceinline__ void stage3z_body(const F3ZArgs<T>& a, ZSlab<T>& sl, 
 #pragma unroll
                 for (int s = 0; s < 4; ++s) t[4 * cc + s] = q4[s] * d[cc];
             // wave total of the 16 values: lane L ends with value L >> 2
-            double b8[8], b4[4], b2[2], b1[1];
-            halve_xor<16>(t, b8, 32, lane & 32);
-            halve_xor<8>(b8, b4, 16, lane & 16);
-            halve_xor<4>(b4, b2, 8, lane & 8);
-            halve_xor<2>(b2, b1, 4, lane & 4);
-            double tot = b1[0] + __shfl_xor(b1[0], 2);
-            tot = tot + __shfl_xor(tot, 1);
+            const double tot = wave_sum16(t);
             if (row_ok && (lane & 3) == 0)
                 a.part[(((long long)i * nz + j) * a.nch + ch) * kNMom3 + rec_pos(grp, lane >> 2)] = tot;
         }

@@ -34,6 +34,7 @@
 #include <hip/hip_runtime.h>
 
 #include "nuosc_dev.h"
+#include "nuosc_lanes.h"
 
 #include <algorithm>
 #include <cstdint>
@@ -273,7 +274,9 @@ __device__ __forceinline__ void halve_c(const double* in, const double* ine, dou
 template <int L, bool COMP>
 __device__ __forceinline__ double group_sum16(const double (&a)[16], const double (&ae)[16], int lane) {
     constexpr int M1 = L / 2, M2 = L / 4, M3 = L / 8, M4 = L / 16;
-    if constexpr (!COMP) {
+    if constexpr (!COMP && L == 64) {
+        return wave_sum16(a);   // the same tree with permlane swaps and DPP moves
+    } else if constexpr (!COMP) {
         double b[8], c[4], d[2], e[1];
         halve<M1>(a, b, 8, lane & M1);
         halve<M2>(b, c, 4, lane & M2);
