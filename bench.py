@@ -476,7 +476,7 @@ def measure_box(N, a, precision):
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": ("k_stage3z stage 2" if precision == 64 else "k_stage3 stage 2") if three else "k_stage stage 2",
+            "kernel": "k_stage3z stage 2" if three else "k_stage stage 2",
             "achieved": s2_gbs,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

@@ -3,11 +3,13 @@
 // 189-277; vacuum Hamiltonian :543-569).
 //
 // Kernels over the state copies (the layout of nuosc_dev.h with nf = 18):
-//   k_stage3z   FP64 boxes with nx > 1: the stage as a z march, 8 wavefronts
-//               over 4 x rows of one 64-direction chunk, the 18 fields split
+//   k_stage3z   grids with nx > 1: the stage as a z march, 8 wavefronts over
+//               4 x rows of one 64-direction chunk, the 18 fields split
 //               between two groups of waves along the moment vectors (below);
-//   k_stage3    the FP32 form and one-row grids: one wavefront per (site, 64 directions), four x rows per
-//               block: the site's 18 centre values and 8 moment projections
+//   k_stage3    one-row grids (nx = 1), and boxes whose 4-row block span
+//               passes 32-bit buffer offsets: one wavefront per (site, 64
+//               directions), four x rows (one row: four direction chunks)
+//               per block: the site's 18 centre values and 8 moment projections
 //               give the interaction and vacuum terms; each field's x
 //               neighbours come from the block's LDS (the waves' own rows and
 //               one edge row loaded by each wave), its z neighbours straight
@@ -21,7 +23,7 @@
 //               init or upload, and for calRHS).
 // All sums run in a fixed order, so a run's result does not depend on the
 // launch.
-// The logical block order walks x quads inside z tiles of 16 sites, so the
+// k_stage3's logical block order walks x quads inside z tiles of 16 sites, so the
 // z neighbours of a block were read a block or two earlier and its x
 // neighbours (the next quads) 16 blocks earlier, on the same XCD, and most
 // stencil re-reads hit L2 (HBM reads 1.3-1.6x one pass of the stage input at
@@ -365,9 +367,13 @@ __global__ __launch_bounds__(64 * kF3Rows) __attribute__((amdgpu_waves_per_eu(4)
 // wave writes its row's centre values of its fields, and waves 0-3 (group A)
 // and 4-7 (group B) load the 4 rows beyond the block's edges for their fields
 // one step ahead.  The slab is double-buffered by the parity of the z step
-// (147 KB, one block per CU), so one barrier per step orders both the
-// writes before the reads and the reads before the rewrite two steps later
-// (8% faster per step than a single slab with a second barrier).
+// (FP64 147 KB: one block per CU, 2 waves per SIMD; FP32 74 KB: two blocks,
+// 4 waves per SIMD), so one barrier per step orders both the writes before
+// the reads and the reads before the rewrite two steps later (8% faster per
+// FP64 step than a single slab with a second barrier).  u (or A) of the
+// step is loaded after the interaction terms, right before the fields'
+// loop: fewer registers live across the terms (FP32: 6-8 VGPRs spilled
+// instead of 9-11; 2-10% faster per step in both precisions).
 // ---------------------------------------------------------------------------
 constexpr int kZW = 8;          // waves per block
 constexpr int kZRows = 4;       // x rows per block
@@ -488,22 +494,13 @@ __device__ __forceinline__ void stage3z_body(const F3ZArgs<T>& a, ZSlab<T>& sl, 
     }
     for (int j = j0; j < j1; ++j) {
         const unsigned so = (unsigned)(j + zoff) * siteb;
-        // ---- loads for this and the next step: own row z+2, u or A of z,
-        // edge row z+1
+        // ---- loads: own row z+2 (this step), edge row z+1 (next step)
         const __amdgpu_buffer_rsrc_t ry = rs(a.yin);
         const unsigned s2 = zrow(j + 2) * siteb, s1 = zrow(j + 1) * siteb;
 #pragma unroll
         for (int q = 0; q < NF; ++q)
             win[q][4] = zld<T>(ry, vo, s2 + (unsigned)kGroupField[grp][q] * rowb);
         __builtin_amdgcn_sched_barrier(0);
-        T uc[NF];
-        if constexpr (STAGE == 2 || STAGE == 3 || STAGE == 6 || STAGE == 7 || STAGE == 4) {
-            const __amdgpu_buffer_rsrc_t ru = rs(STAGE == 4 ? a.w : a.u);
-#pragma unroll
-            for (int q = 0; q < NF; ++q)
-                uc[q] = zld<T>(ru, vo, so + (unsigned)kGroupField[grp][q] * rowb);
-            __builtin_amdgcn_sched_barrier(0);
-        }
         // ---- LDS: own centre values and the edge rows of z (loaded last step)
         T (*X)[kZSlots][64] = sl.x[j & 1];
 #pragma unroll
@@ -533,6 +530,13 @@ __device__ __forceinline__ void stage3z_body(const F3ZArgs<T>& a, ZSlab<T>& sl, 
         }
         T I[kNVar3], V[kNVar3];
         local_terms3<T>(c, P, (T)a.mu, (T)a.pmo, h, I, V);
+        T uc[NF];
+        if constexpr (STAGE == 2 || STAGE == 3 || STAGE == 6 || STAGE == 7 || STAGE == 4) {
+            const __amdgpu_buffer_rsrc_t ru = rs(STAGE == 4 ? a.w : a.u);
+#pragma unroll
+            for (int q = 0; q < NF; ++q)
+                uc[q] = zld<T>(ru, vo, so + (unsigned)kGroupField[grp][q] * rowb);
+        }
         const __amdgpu_buffer_rsrc_t ro = rs(a.yout);
         const __amdgpu_buffer_rsrc_t rw = rs(a.w);
         double t[16];
@@ -606,7 +610,7 @@ __device__ __forceinline__ void stage3z_body(const F3ZArgs<T>& a, ZSlab<T>& sl, 
 // waves 0-3 run group A's fields, waves 4-7 group B's: both execute the same
 // number of barriers per z step, which is what the block barrier counts
 template <typename T, int STAGE>
-__global__ __launch_bounds__(64 * kZW) __attribute__((amdgpu_waves_per_eu(2)))
+__global__ __launch_bounds__(64 * kZW) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 2 : 4)))
 void k_stage3z(const F3ZArgs<T> a) {
     __shared__ ZSlab<T> sl;
     const int lb = xcd_remap(blockIdx.x, gridDim.x);
@@ -671,10 +675,7 @@ hipError_t launch_stage3(int stage, const Geometry& g, const Physics& ph, const 
                          const double* mom, double* part, double cw, double cy, hipStream_t s, double ca,
                          double cb) {
     if (g.nf != kNVar3 || (stage != 0 && !part)) return hipErrorInvalidValue;
-    // FP64 boxes take the z march; the FP32 form keeps k_stage3 (the march
-    // held to 4 waves per SIMD spills, and was 4% slower per step)
-    if constexpr (sizeof(T) == 8) {
-      if (z_march_ok(g, (int)sizeof(T))) {
+    if (z_march_ok(g, (int)sizeof(T))) {
         F3ZArgs<T> z{};
         z.g = g;
         z.yin = (unsigned long long)yin;
@@ -711,7 +712,6 @@ hipError_t launch_stage3(int stage, const Geometry& g, const Physics& ph, const 
             case 7: return launch3z<T, 7>(z, (unsigned)nb, s);
             default: return hipErrorInvalidValue;
         }
-      }
     }
     F3Args<T> a{};
     a.g = g;
