@@ -96,9 +96,9 @@ void batch_detach(nuosc_batch* b, nuosc_sim* s);
 bool is64(const nuosc_sim* s) { return s->cfg.precision == NUOSC_PREC_FP64; }
 
 // Strides of the four state copies (separate allocations, site records of
-// nf nvp elements); ok = false: the 2-flavour stage kernel's per-wave range
-// of x rows or the moment records do not fit 32-bit buffer offsets (the
-// 3-flavour kernels address with 64-bit pointers).
+// nf nvp elements); ok = false: the x rows one stage block addresses (2
+// flavours: also the moment records; 3 flavours with nx > 1: the z march's
+// block of rows) do not fit 32-bit buffer offsets.
 struct CopyLayout {
     long long srec, xrow, elems;   // strides and elements allocated for all four copies
     bool ok;
@@ -113,6 +113,7 @@ CopyLayout copy_layout(int nx, int nz, int nzs, int nvp, int nch, int esize, int
     gq.srec = (long long)nf * nvp;
     gq.xrow = (long long)nzs * gq.srec;
     if (nf == kNVar && !buffer_addressable(gq, esize)) return CopyLayout{0, 0, 0, false};
+    if (nf == kNVar3 && !flavor3_addressable(gq, esize)) return CopyLayout{0, 0, 0, false};
     return CopyLayout{gq.srec, gq.xrow, 4 * copy_elems(gq), true};
 }
 

@@ -156,6 +156,9 @@ StagePlan plan_stage_zlen(const Geometry& g, int zlen);
 // them) and the moment records through buffer descriptors: each range must
 // span fewer than 4 GiB.
 bool buffer_addressable(const Geometry& g, int esize);
+// The same for the 3-flavour z march (boxes with nx > 1; one-row grids take
+// a kernel with 64-bit addressing).
+bool flavor3_addressable(const Geometry& g, int esize);
 
 template <typename T>
 hipError_t launch_stage(int stage, const Geometry& g, const Physics& ph, const StagePlan& pl,

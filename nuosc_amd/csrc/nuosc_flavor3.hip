@@ -6,10 +6,10 @@
 //   k_stage3z   grids with nx > 1: the stage as a z march, 8 wavefronts over
 //               4 x rows of one 64-direction chunk, the 18 fields split
 //               between two groups of waves along the moment vectors (below);
-//   k_stage3    one-row grids (nx = 1), and boxes whose 4-row block span
-//               passes 32-bit buffer offsets: one wavefront per (site, 64
-//               directions), four x rows (one row: four direction chunks)
-//               per block: the site's 18 centre values and 8 moment projections
+//   k_stage3    one-row grids (nx = 1; written for 4 x rows per block,
+//               launched in its one-row form): one wavefront per (site, 64
+//               directions), four direction chunks of the row per block:
+//               the site's 18 centre values and 8 moment projections
 //               give the interaction and vacuum terms; each field's x
 //               neighbours come from the block's LDS (the waves' own rows and
 //               one edge row loaded by each wave), its z neighbours straight
@@ -631,10 +631,15 @@ __global__ __launch_bounds__(256) void k_moment_totals3(const double* __restrict
     mom[k] = s;
 }
 
+// k_stage3 runs one-row grids only (ONE = true): boxes take the z march, and
+// boxes it cannot address are refused at setup (flavor3_addressable).  The
+// one-row form keeps the multi-row kernel's LDS staging and barriers (their
+// values unused): a copy without them, and one with scheduling barriers in
+// their place, measured 8-11% slower per 1x4096x642 column step.
 template <typename T, int STAGE>
 hipError_t launch3(const F3Args<T>& a, unsigned nb, hipStream_t s) {
-    if (a.one_row) hipLaunchKernelGGL((k_stage3<T, STAGE, true>), dim3(nb), dim3(64 * kF3Rows), 0, s, a);
-    else hipLaunchKernelGGL((k_stage3<T, STAGE, false>), dim3(nb), dim3(64 * kF3Rows), 0, s, a);
+    if (!a.one_row) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_stage3<T, STAGE, true>), dim3(nb), dim3(64 * kF3Rows), 0, s, a);
     return hipGetLastError();
 }
 
@@ -644,15 +649,15 @@ hipError_t launch3z(const F3ZArgs<T>& a, unsigned nb, hipStream_t s) {
     return hipGetLastError();
 }
 
-// The z march takes boxes with nx > 1 whose x rows addressed by one block
-// (4 rows, 3 ghost rows each side, the spare row and one of margin) span less
-// than 4 GiB (buffer offsets are 32-bit); other grids use k_stage3.
-bool z_march_ok(const Geometry& g, int esize) {
-    const long long span = (long long)(kZRows + 2 * kXg + 2) * g.xrow * esize;
-    return g.nx > 1 && span < (1ll << 32) - 1;
-}
-
 }  // namespace
+
+// The z march (boxes with nx > 1) addresses the x rows of one block (4 rows,
+// 3 ghost rows each side, the spare row and one of margin) through 32-bit
+// buffer offsets: they must span less than 4 GiB.
+bool flavor3_addressable(const Geometry& g, int esize) {
+    const long long span = (long long)(kZRows + 2 * kXg + 2) * g.xrow * esize;
+    return g.nx == 1 || span < (1ll << 32) - 1;
+}
 
 template <typename T>
 hipError_t launch_moments3(const Geometry& g, const T* y, double* mom, hipStream_t s) {
@@ -675,7 +680,8 @@ hipError_t launch_stage3(int stage, const Geometry& g, const Physics& ph, const 
                          const double* mom, double* part, double cw, double cy, hipStream_t s, double ca,
                          double cb) {
     if (g.nf != kNVar3 || (stage != 0 && !part)) return hipErrorInvalidValue;
-    if (z_march_ok(g, (int)sizeof(T))) {
+    if (g.nx > 1) {
+        if (!flavor3_addressable(g, (int)sizeof(T))) return hipErrorInvalidValue;
         F3ZArgs<T> z{};
         z.g = g;
         z.yin = (unsigned long long)yin;

@@ -4,6 +4,7 @@ import os
 import re
 
 import numpy as np
+import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -56,3 +57,22 @@ def test_gpu_kernels_are_gfx950_code_objects():
     data = open(os.path.join(ROOT, "nuosc_amd", "libnuosc_hip.so"), "rb").read()
     assert b"gfx950" in data
     assert b"k_stage" in data
+
+
+def test_estimate_refuses_unaddressable_boxes(nuosc_mod):
+    """Host-only sizing (no GPU): the stage kernels address one block's x rows
+    through 32-bit buffer offsets, so a box whose stored x row is too long is
+    refused up front, for both flavour counts; one x row of the same length is
+    accepted with 3 flavours (that kernel addresses with 64-bit pointers)."""
+    N = nuosc_mod
+    # 642 directions, FP64: a stored x row is nz * nf * 648 * 8 bytes
+    ok2 = N.estimate_device_bytes(nv=8, x0=0.0, x1=0.8, z0=0.0, z1=409.6, dx=0.1, dz=0.1, nflavor=2)   # nz 4096
+    assert ok2 > 0
+    with pytest.raises(N.NuOscError):   # 12 rows x 16384 x 8 x 648 x 8 B > 4 GiB
+        N.estimate_device_bytes(nv=8, x0=0.0, x1=0.8, z0=0.0, z1=1638.4, dx=0.1, dz=0.1, nflavor=2)
+    ok3 = N.estimate_device_bytes(nv=8, x0=0.0, x1=0.8, z0=0.0, z1=204.8, dx=0.1, dz=0.1, nflavor=3)   # nz 2048
+    assert ok3 > 0
+    with pytest.raises(N.NuOscError):   # 12 rows x 4096 x 18 x 648 x 8 B > 4 GiB
+        N.estimate_device_bytes(nv=8, x0=0.0, x1=0.8, z0=0.0, z1=409.6, dx=0.1, dz=0.1, nflavor=3)
+    # the same z length in one x row: no block span limit for 3 flavours
+    assert N.estimate_device_bytes(nv=8, x0=0.0, x1=0.1, z0=0.0, z1=409.6, dx=0.1, dz=0.1, nflavor=3) > 0
