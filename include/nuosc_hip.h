@@ -149,7 +149,11 @@ const char* nuosc_build_info(void);
 int nuosc_vgrid(int nn, int mode, double* vx, double* vy, double* vz, double* vw);
 
 /* NuOsc::NuOsc (nuosc_class.cpp:454-534): geometry, velocity grid,
- * allocation.  The state is zero until nuosc_fill_init or an upload. */
+ * allocation.  The state is zero until nuosc_fill_init or an upload.
+ * NUOSC_EINVAL also for grids the stage kernels cannot address: their 32-bit
+ * buffer offsets span one block's stored x rows (12 rows of nz * nf * nvp
+ * values; 2 flavours: also the moment records) and must stay under 4 GiB
+ * (one x row of 3 flavours is not limited). */
 int nuosc_create(const nuosc_config* cfg, nuosc_sim** out);
 /* NuOsc::~NuOsc (nuosc_class.h:136-146) */
 int nuosc_destroy(nuosc_sim* sim);
