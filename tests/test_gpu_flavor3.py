@@ -13,6 +13,8 @@ nuosc_class.cpp:28-82, 189-277, 543-569) against the oracle:
   * 50 RK4 steps of the FP32 form against FP64: Pee and fields within
     4 * 50 * 2^-24 * 2 = 2.4e-5 (of max|field|): one FP32 rounding per stage
     store, never cancelling, amplified at most twofold;
+  * the FP32 form on open z (6 x rows, RK4) and with SSP-RK3, 20 steps each,
+    against FP64 under the same kind of bound;
   * renormalising runs, 10 steps: relative L-inf <= 1e-10 per field;
   * hipGraph replay bitwise equal to eager launches; per-stage timing;
   * the CLI with --nflavor 3, and the arguments the path refuses.
@@ -272,5 +274,39 @@ def test_flavor3_fp32_tracks_fp64(nuosc_mod):
     assert np.max(np.abs(sa - sb)) <= bound * np.max(np.abs(sa))
     assert np.any(sa[8:])                        # the vacuum term populated the tau fields
     assert abs(a.analysis().surv - b.analysis().surv) <= bound
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("case", ["open_rk4", "periodic_srk3"])
+def test_flavor3_fp32_tracks_fp64_open_srk3(nuosc_mod, case):
+    """The FP32 z march on open z (6 x rows, 20 RK4 steps) and with SSP-RK3
+    (12 x 16, 20 steps) against FP64 from the same state.  Bound as in
+    test_flavor3_fp32_tracks_fp64: s FP32 stage stores per step (RK4 4,
+    SSP-RK3 3), each rounding every field by at most 2^-24 relative, never
+    cancelling, amplified at most twofold: s * 20 * 2^-24 * 2 of max|field|."""
+    N = nuosc_mod
+    if case == "open_rk4":
+        c = dict(nv=2, x0=0.0, x1=0.6, z0=-3.2, z1=3.2, dx=0.1, dz=0.1, CFL=0.4, ko=0.05, mu=1.0, pmo=0.3,
+                 bc=N.BC_OPEN_Z, zc=-1.0)
+        init = dict(ipt=0, alpha=0.9, eps0=1e-2, sigma=0.5, lnue=0.6, lnueb=0.5)
+        stores = 4
+    else:
+        c = cfg_for(12, 16, 4, ko=0.05, pmo=0.5)
+        init = dict(ipt=5, alpha=0.9, eps0=1e-3, sigma=1.0, lnue=0.6, lnueb=0.5, seed=11)
+        stores = 3
+    a = N.NuOsc(**c, nflavor=3)
+    b = N.NuOsc(**c, nflavor=3, precision=N.PREC_FP32)
+    assert a.shape[0] > 1                        # a box: the z-march kernel
+    for g in (a, b):
+        g.fillInitValue(**init)
+        if case == "open_rk4":
+            g.step_rk4(20)
+        else:
+            g.step_srk3(20)
+    sa, sb = a.get_state(), b.get_state()
+    bound = stores * 20 * 2.0 ** -24 * 2
+    assert np.all(np.isfinite(sb))
+    assert np.max(np.abs(sa - sb)) <= bound * np.max(np.abs(sa)), np.max(np.abs(sa - sb)) / np.max(np.abs(sa))
     a.close()
     b.close()
