@@ -118,10 +118,11 @@ CopyLayout copy_layout(int nx, int nz, int nzs, int nvp, int nch, int esize, int
 }
 
 void drop_graph(nuosc_sim* s) {
-    if (s->graph_exec) hipGraphExecDestroy(s->graph_exec);
-    if (s->graph) hipGraphDestroy(s->graph);
-    if (s->graph3_exec) hipGraphExecDestroy(s->graph3_exec);
-    if (s->graph3) hipGraphDestroy(s->graph3);
+    // (teardown: errors of the destroy calls leave nothing to recover)
+    if (s->graph_exec) (void)hipGraphExecDestroy(s->graph_exec);
+    if (s->graph) (void)hipGraphDestroy(s->graph);
+    if (s->graph3_exec) (void)hipGraphExecDestroy(s->graph3_exec);
+    if (s->graph3) (void)hipGraphDestroy(s->graph3);
     s->graph_exec = nullptr;
     s->graph = nullptr;
     s->graph3_exec = nullptr;
@@ -131,8 +132,9 @@ void drop_graph(nuosc_sim* s) {
 int ensure_tmp(nuosc_sim* s, size_t bytes) {
     if (s->tmp_bytes >= bytes) return NUOSC_OK;
     if (s->d_tmp) {
-        hipStreamSynchronize(s->stream);
-        hipFree(s->d_tmp);
+        // the staging buffer may still be read by queued copies
+        if (hipStreamSynchronize(s->stream) != hipSuccess) return fail(s, NUOSC_EHIP, "stream sync failed");
+        (void)hipFree(s->d_tmp);
         s->device_bytes -= (long long)s->tmp_bytes;
         s->d_tmp = nullptr;
         s->tmp_bytes = 0;
@@ -644,15 +646,17 @@ int nuosc_destroy(nuosc_sim* s) {
     // a member leaves its batch before its buffers go (the batch's tables
     // and captured step point at them)
     if (s->batch) batch_detach(s->batch, s);
-    if (s->stream) hipStreamSynchronize(s->stream);
+    // (teardown: a failed call here leaves nothing to recover, so the
+    // remaining resources are released regardless)
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
     drop_graph(s);
     void* bufs[] = {s->st_u, s->st_w, s->st_ya, s->st_yb, s->mom_part, s->mom_u, s->mom_a, s->mom_b,
                     s->d_vgrid, s->d_g0, s->d_part, s->d_ana, s->d_tmp, s->d_time};
     for (void* p : bufs)
-        if (p) hipFree(p);
-    if (s->ev0) hipEventDestroy(s->ev0);
-    if (s->ev1) hipEventDestroy(s->ev1);
-    if (s->stream) hipStreamDestroy(s->stream);
+        if (p) (void)hipFree(p);
+    if (s->ev0) (void)hipEventDestroy(s->ev0);
+    if (s->ev1) (void)hipEventDestroy(s->ev1);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
     return NUOSC_OK;
 }
@@ -804,7 +808,7 @@ int nuosc_step_rk4(nuosc_sim* s, int nsteps) {
             hipGraph_t gr = nullptr;
             hipError_t e = hipStreamEndCapture(s->stream, &gr);
             if (rc) {
-                if (gr) hipGraphDestroy(gr);
+                if (gr) (void)hipGraphDestroy(gr);
                 return rc;
             }
             if (e != hipSuccess) return fail(s, NUOSC_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
@@ -842,7 +846,7 @@ int nuosc_step_srk3(nuosc_sim* s, int nsteps) {
             hipGraph_t gr = nullptr;
             hipError_t e = hipStreamEndCapture(s->stream, &gr);
             if (rc) {
-                if (gr) hipGraphDestroy(gr);
+                if (gr) (void)hipGraphDestroy(gr);
                 return rc;
             }
             if (e != hipSuccess) return fail(s, NUOSC_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
@@ -899,13 +903,14 @@ int nuosc_profile_stages(nuosc_sim* s, int nsteps, double* stage_ms) {
             rc = fail(s, NUOSC_EHIP, "event sync failed");
             break;
         }
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < 4 && rc == NUOSC_OK; ++k) {
             float f = 0;
-            hipEventElapsedTime(&f, ev[2 * k], ev[2 * k + 1]);
+            if (hipEventElapsedTime(&f, ev[2 * k], ev[2 * k + 1]) != hipSuccess)
+                rc = fail(s, NUOSC_EHIP, "event timing failed");
             acc[k] += f;
         }
     }
-    for (auto& e : ev) hipEventDestroy(e);
+    for (auto& e : ev) (void)hipEventDestroy(e);
     for (int k = 0; k < 4; ++k) stage_ms[k] = acc[k] / nsteps;
     return rc;
 }
