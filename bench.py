@@ -66,6 +66,8 @@ def parse():
     p.add_argument("--no-extras", action="store_true",
                    help="box FP64: skip the FP32 and batched legs of the default line")
     p.add_argument("--no-autotune", action="store_true", help="box: keep the default z-segment plan")
+    p.add_argument("--zlen", type=int, default=0,
+                   help="box, 2 flavours: this z-segment length instead of the autotuned one (reproducible profiles)")
     p.add_argument("--cpu-seconds", type=float, default=12.0)
     p.add_argument("--nflavor", type=int, default=2, choices=(2, 3), help="box: 2 (8 fields) or 3 (18 fields)")
     p.add_argument("--workload", default="box", choices=("box", "batched", "served", "precision"),
@@ -431,7 +433,11 @@ def measure_box(N, a, precision):
         g.fillInitValue(ipt=5, alpha=0.9, eps0=1e-6, sigma=1.0, lnue=0.6, lnueb=0.5, seed=12345)
     nx, nz, nv = g.shape
     upd = nx * nz * nv
-    zlen = g.plan()[0] if a.no_autotune else g.autotune(trials=2)[0]
+    if a.zlen and not three:
+        g.set_zlen(a.zlen)
+        zlen = a.zlen
+    else:
+        zlen = g.plan()[0] if a.no_autotune else g.autotune(trials=2)[0]
 
     g.step_rk4(a.warmup)
     g.synchronize()
