@@ -23,6 +23,10 @@ Prints one JSON line:
   batched        (default box run) the 64-run batched scan (FP32), 20 steps:
                  aggregate updates/s together and one after another, the
                  speed-up, and each mode's HBM fraction at 384 B per update
+  served         (default box run) the run server, FP32: 20 s of Poisson
+                 0.35/s arrivals of runs drawn like the batched scan's (grids
+                 64..384 squared, steps U[100,1000]): p50/p99 request latency
+                 and queueing delay, aggregate updates/s
   cpu_baseline   the CPU oracle on a 128 x 128 x 642 crop of the same physics,
                  timed on this host's cores, median of 3 samples (a reported
                  baseline), with its reference-equivalent rate
@@ -188,19 +192,27 @@ def bench_batched(a):
 
 def bench_served(a):
     """Poisson arrivals into the run server; latency percentiles."""
+    print(json.dumps(served_leg(a.rate, a.duration, a.steps, a.precision, a.grids)), flush=True)
+
+
+def served_leg(rate, duration, steps, precision, grids=None):
+    """Poisson arrivals (rate per second, for duration seconds) of runs with
+    grids drawn from `grids` squared, 162/642 directions and U[steps,
+    10 steps] RK4 steps into the run server; returns request latency and
+    queueing percentiles and the aggregate rate."""
     import numpy as np
 
     import nuosc_amd as N
-    prec = N.PREC_FP64 if a.precision == 64 else N.PREC_FP32
+    prec = N.PREC_FP64 if precision == 64 else N.PREC_FP32
     rng = np.random.default_rng(11)
-    grids = tuple(int(x) for x in (a.grids or "64,128,192,256,320,384").split(","))
+    grids = tuple(int(x) for x in (grids or "64,128,192,256,320,384").split(","))
     srv = N.Server(device=0, precision=prec, hbm_budget_bytes=200 << 30, max_tick_steps=10)
     t_start = time.perf_counter()
     n, k = 0, 0
-    next_t = rng.exponential(1.0 / a.rate)
+    next_t = rng.exponential(1.0 / rate)
     submitted = {}
     last_note = 0.0
-    while next_t < a.duration:
+    while next_t < duration:
         now = time.perf_counter() - t_start
         if now - last_note > 30:
             print(f"served: {k} requests submitted at {now:.1f} s", file=sys.stderr, flush=True)
@@ -209,11 +221,11 @@ def bench_served(a):
             time.sleep(min(0.05, next_t - now))
             continue
         geo, init = scan_run(rng, k, grids=grids)
-        nsteps = int(rng.integers(a.steps, 10 * a.steps + 1))
+        nsteps = int(rng.integers(steps, 10 * steps + 1))
         rid = srv.submit(nsteps=nsteps, ana_every=max(1, nsteps // 20), **geo, **init)
         submitted[rid] = nsteps
         k += 1
-        next_t += rng.exponential(1.0 / a.rate)
+        next_t += rng.exponential(1.0 / rate)
     print(f"served: {len(submitted)} requests submitted in {time.perf_counter() - t_start:.1f} s", file=sys.stderr,
           flush=True)
     lat, queue_wait, bad = [], [], 0
@@ -233,16 +245,16 @@ def bench_served(a):
         "metric": "served scan: request latency and aggregate updates/s",
         "value": st.updates / wall,
         "unit": "updates/s",
-        "requests": len(submitted), "failed": bad, "rate_per_s": a.rate, "arrival_window_s": a.duration,
+        "requests": len(submitted), "failed": bad, "rate_per_s": rate, "arrival_window_s": duration,
         "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99)),
         "queue_p50_s": float(np.percentile(queue_wait, 50)), "queue_p99_s": float(np.percentile(queue_wait, 99)),
         "gpu_busy_frac": st.busy_s / wall,
-        "dtype": "f64" if a.precision == 64 else "f32",
-        "config": {"workload": f"Poisson {a.rate}/s for {a.duration}s, grids drawn from {list(grids)} squared, "
+        "dtype": "f64" if precision == 64 else "f32",
+        "config": {"workload": f"Poisson {rate}/s for {duration}s, grids drawn from {list(grids)} squared, "
                                f"162/642 directions, "
-                               f"steps U[{a.steps},{10 * a.steps}], 200 GB budget"},
+                               f"steps U[{steps},{10 * steps}], 200 GB budget"},
     }
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def bench_precision(a):
@@ -405,6 +417,16 @@ def main():
             "step_roofline_frac": o32["step_roofline"]["frac"], "state_ok": o32["state_ok"],
         }
         out["batched"] = batched_leg(N, a)
+        # p50/p99 run latency of the served scan in a short window (20 s of
+        # Poisson 0.35/s arrivals, about what the GPU sustains with this draw;
+        # steps U[100,1000] so the leg ends within about a minute); the
+        # 600 s run at the full step range is `--workload served --rate 0.35
+        # --duration 600 --precision 32 --steps 200`
+        sv = served_leg(0.35, 20.0, 100, 32)
+        out["served"] = {k: sv[k] for k in ("value", "unit", "requests", "failed", "latency_p50_s",
+                                            "latency_p99_s", "queue_p50_s", "queue_p99_s", "gpu_busy_frac",
+                                            "dtype")}
+        out["served"]["workload"] = sv["config"]["workload"]
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(a.nn, a.dz, a.cpu_seconds, a.nflavor, 1.0 if a.nflavor == 3 else 0.0)
     print(json.dumps(out), flush=True)
