@@ -167,10 +167,15 @@ def bench_batched(a):
     for g in sims:
         b.add(g)
     zbest, _ = b.autotune(trials=2)
-    b.step(1)
-    t0 = time.perf_counter()
-    b.step(a.steps)
-    t_bat = time.perf_counter() - t0
+    chosen = b.launch
+    t_form = {}
+    for lf in (chosen, 1 - chosen):   # the autotuned pair, then the other launch form
+        b.set_launch(lf)
+        b.step(1)
+        t0 = time.perf_counter()
+        b.step(a.steps)
+        t_form[lf] = time.perf_counter() - t0
+    t_bat = t_form[chosen]
     ok = all(g.analysis().status == 0 for g in sims)
     out = {
         "metric": "site*dir RK4 updates/s (batched scan)",
@@ -181,7 +186,10 @@ def bench_batched(a):
         "n_gpus": 1, "steps": a.steps, "runs": a.runs,
         "dtype": "f64" if a.precision == 64 else "f32",
         "config": {"workload": f"{a.runs} runs, grids drawn from {list(grids)} squared, 162/642 directions, "
-                               f"uneven physics", "updates_per_step": upd, "batch_zlen": zbest},
+                               f"uneven physics", "updates_per_step": upd, "batch_zlen": zbest,
+                   "batch_launch": "persistent" if chosen == N.BATCH_PERSISTENT else "grid"},
+        "grid_value": upd * a.steps / t_form[N.BATCH_GRID],
+        "persistent_value": upd * a.steps / t_form[N.BATCH_PERSISTENT],
         "state_ok": ok,
     }
     b.close()
@@ -530,8 +538,10 @@ def measure_box(N, a, precision):
 
 def batched_leg(N, a, runs=64, steps=20):
     """The batched-scan workload (64 uneven runs, FP32) for a few steps:
-    aggregate updates/s stepped all together and one after another, and each
-    mode's HBM fraction at 384 B per FP32 update (12 passes x 8 fields)."""
+    aggregate updates/s stepped all together (at the autotuned z-segment
+    length and launch form; the other launch form is timed too) and one
+    after another, and each mode's HBM fraction at 384 B per FP32 update
+    (12 passes x 8 fields)."""
     import numpy as np
     rng = np.random.default_rng(7)
     specs = [scan_run(rng, k) for k in range(runs)]
@@ -554,10 +564,16 @@ def batched_leg(N, a, runs=64, steps=20):
     for g in sims:
         b.add(g)
     zbest, _ = b.autotune(trials=2)
-    b.step(1)
-    t0 = time.perf_counter()
-    b.step(steps)
-    t_bat = time.perf_counter() - t0
+    chosen = b.launch
+    # the autotuned pair, then the other launch form at the same zlen
+    t_form = {}
+    for lf in (chosen, 1 - chosen):
+        b.set_launch(lf)
+        b.step(1)
+        t0 = time.perf_counter()
+        b.step(steps)
+        t_form[lf] = time.perf_counter() - t0
+    t_bat = t_form[chosen]
     ok = all(g.analysis().status == 0 for g in sims)
     b.close()
     for g in sims:
@@ -568,6 +584,9 @@ def batched_leg(N, a, runs=64, steps=20):
         "value": v_bat, "unit": "updates/s", "sequential_value": v_seq, "speedup_vs_sequential": t_seq / t_bat,
         "hbm_frac": v_bat * bpu / (HBM_PEAK_GBS * 1e9), "sequential_hbm_frac": v_seq * bpu / (HBM_PEAK_GBS * 1e9),
         "bytes_per_update": bpu, "steps": steps, "runs": runs, "dtype": "f32", "batch_zlen": zbest,
+        "batch_launch": "persistent" if chosen == N.BATCH_PERSISTENT else "grid",
+        "grid_value": upd * steps / t_form[N.BATCH_GRID],
+        "persistent_value": upd * steps / t_form[N.BATCH_PERSISTENT],
         "workload": f"{runs} runs, grids drawn from 64..256 squared, 162/642 directions, uneven physics",
         "updates_per_step": upd, "state_ok": ok,
     }

@@ -288,10 +288,21 @@ const char* nuosc_batch_last_error(const nuosc_batch* batch);
  * (default 64, capped at each member's nz; 0 = each member's own plan).  A
  * common length gives every block of a batched launch the same work.
  * batch_autotune times the batched stage-1 kernel for zlen in {16, ..., 256}
- * (trials launches each, the fastest kept; only the RK4 scratch copies are
- * written) and keeps the best.  Results do not depend on zlen. */
+ * and both launch forms (trials launches each, the fastest kept; only the
+ * RK4 scratch copies are written) and keeps the best pair (the launch form
+ * it chose: nuosc_batch_get_launch).  Results do not depend on zlen or the
+ * launch form. */
 int nuosc_batch_set_zlen(nuosc_batch* batch, int zlen);
 int nuosc_batch_autotune(nuosc_batch* batch, int trials, int* best_zlen, double* best_ms);
+/* Launch form of the batched stage kernels: NUOSC_BATCH_GRID (default: one
+ * block per logical block of every member) or NUOSC_BATCH_PERSISTENT (one
+ * grid of as many blocks as the GPU holds at once, drawing logical blocks
+ * from per-XCD work queues until they are empty).  get_launch returns the
+ * form, or -1 for a NULL batch. */
+#define NUOSC_BATCH_GRID 0
+#define NUOSC_BATCH_PERSISTENT 1
+int nuosc_batch_set_launch(nuosc_batch* batch, int launch);
+int nuosc_batch_get_launch(const nuosc_batch* batch);
 
 /* Host-only: device bytes a handle created from cfg would hold (admission
  * control for the serving loop). */

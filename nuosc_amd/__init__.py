@@ -30,6 +30,8 @@ NUOSC_OK, NUOSC_EINVAL, NUOSC_EHIP, NUOSC_ENOMEM, NUOSC_EUNSTABLE, NUOSC_ENODEV,
 PREC_FP64, PREC_FP32 = 64, 32
 VGRID_VORONOI, VGRID_EQUAL = 0, 1
 BC_PERIODIC, BC_OPEN_Z = 0, 1
+# launch forms of the batched stage kernels (NUOSC_BATCH_*)
+BATCH_GRID, BATCH_PERSISTENT = 0, 1
 # advection schemes (NUOSC_ADV_*): the reference's default and its ADVEC_* variants
 ADV_CFD4, ADV_CFD2, ADV_UPWIND2, ADV_LOPSIDED, ADV_MUSCL, ADV_OFF = range(6)
 
@@ -140,6 +142,8 @@ SIGNATURES = {
     "nuosc_batch_last_error": ([_vp], C.c_char_p),
     "nuosc_batch_set_zlen": ([_vp, C.c_int], C.c_int),
     "nuosc_batch_autotune": ([_vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)], C.c_int),
+    "nuosc_batch_set_launch": ([_vp, C.c_int], C.c_int),
+    "nuosc_batch_get_launch": ([_vp], C.c_int),
     "nuosc_estimate_device_bytes": ([C.POINTER(Config), C.POINTER(C.c_longlong)], C.c_int),
     "nuosc_server_create": ([C.c_int, C.c_int, C.c_longlong, C.c_int, C.POINTER(_vp)], C.c_int),
     "nuosc_server_submit": ([_vp, C.POINTER(Request), C.POINTER(C.c_longlong)], C.c_int),
@@ -425,10 +429,20 @@ class Batch:
         self._check(lib().nuosc_batch_set_zlen(self._h, int(zlen)))
 
     def autotune(self, trials=2):
-        """Pick the fastest shared z-segment length; returns (zlen, stage-1 ms)."""
+        """Pick the fastest shared z-segment length and launch form; returns
+        (zlen, stage-1 ms); the form chosen is `launch`."""
         z, ms = C.c_int(), C.c_double()
         self._check(lib().nuosc_batch_autotune(self._h, int(trials), C.byref(z), C.byref(ms)))
         return z.value, ms.value
+
+    def set_launch(self, launch):
+        """BATCH_GRID (one block per logical block) or BATCH_PERSISTENT
+        (resident blocks drawing logical blocks from per-XCD work queues)."""
+        self._check(lib().nuosc_batch_set_launch(self._h, int(launch)))
+
+    @property
+    def launch(self):
+        return lib().nuosc_batch_get_launch(self._h)
 
     def close(self):
         if getattr(self, "_h", None):

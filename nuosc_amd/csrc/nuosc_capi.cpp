@@ -1187,8 +1187,11 @@ struct nuosc_batch {
     // One length gives every block of the batched launch the same amount of
     // work, so the contiguous block ranges of the 8 XCDs stay balanced.
     int zlen = 64;
+    int launch = kBatchGrid;   // NUOSC_BATCH_GRID or NUOSC_BATCH_PERSISTENT
     std::string err;
 };
+static_assert(NUOSC_BATCH_GRID == kBatchGrid && NUOSC_BATCH_PERSISTENT == kBatchPersistent,
+              "launch forms of the header and the kernels agree");
 
 namespace {
 int bfail(nuosc_batch* b, int code, const std::string& m) {
@@ -1234,7 +1237,7 @@ int batch_tables(nuosc_batch* b) {
         bm.dt = m->dt;
         ms.push_back(bm);
     }
-    hipError_t e = batch_build(b->tables, ms, is64(b->members[0]) ? 8 : 4, b->stream);
+    hipError_t e = batch_build(b->tables, ms, is64(b->members[0]) ? 8 : 4, b->launch, b->stream);
     if (e != hipSuccess) return bfail(b, NUOSC_EHIP, std::string("batch tables: ") + hipGetErrorString(e));
     return NUOSC_OK;
 }
@@ -1351,6 +1354,16 @@ int nuosc_batch_set_zlen(nuosc_batch* b, int zlen) {
     return NUOSC_OK;
 }
 
+int nuosc_batch_set_launch(nuosc_batch* b, int launch) {
+    if (!b || (launch != NUOSC_BATCH_GRID && launch != NUOSC_BATCH_PERSISTENT)) return NUOSC_EINVAL;
+    (void)hipStreamSynchronize(b->stream);
+    b->launch = launch;
+    b->dirty = true;
+    return NUOSC_OK;
+}
+
+int nuosc_batch_get_launch(const nuosc_batch* b) { return b ? b->launch : -1; }
+
 int nuosc_batch_autotune(nuosc_batch* b, int trials, int* best_zlen, double* best_ms) {
     if (!b || trials < 1) return NUOSC_EINVAL;
     if (b->members.empty()) return bfail(b, NUOSC_EINVAL, "empty batch");
@@ -1359,14 +1372,18 @@ int nuosc_batch_autotune(nuosc_batch* b, int trials, int* best_zlen, double* bes
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
         return bfail(b, NUOSC_EHIP, "event creation failed");
-    const int keep = b->zlen;
-    const std::vector<int> cand = {16, 32, 64, 128, 256};
+    const int keep = b->zlen, keep_launch = b->launch;
+    // (z-segment length, launch form) pairs
+    std::vector<std::pair<int, int>> cand;
+    for (int zl : {16, 32, 64, 128, 256})
+        for (int lf : {NUOSC_BATCH_GRID, NUOSC_BATCH_PERSISTENT}) cand.push_back({zl, lf});
     std::vector<float> tmin(cand.size(), 1e30f);
     // candidates in turn (tables rebuilt per candidate), trials + 1 rounds,
     // the first a warm-up, so clock drift during the search favours none
     for (int t = 0; t < trials + 1 && rc == NUOSC_OK; ++t)
         for (size_t k = 0; k < cand.size() && rc == NUOSC_OK; ++k) {
-            b->zlen = cand[k];
+            b->zlen = cand[k].first;
+            b->launch = cand[k].second;
             rc = batch_tables(b);
             if (rc) break;
             (void)hipEventRecord(e0, b->stream);
@@ -1378,7 +1395,10 @@ int nuosc_batch_autotune(nuosc_batch* b, int trials, int* best_zlen, double* bes
                 break;
             }
             float ms = 0;
-            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+                rc = bfail(b, NUOSC_EHIP, "autotune timing failed");
+                break;
+            }
             if (t > 0) tmin[k] = std::min(tmin[k], ms);
         }
     (void)hipEventDestroy(e0);
@@ -1386,11 +1406,12 @@ int nuosc_batch_autotune(nuosc_batch* b, int trials, int* best_zlen, double* bes
     size_t kb = 0;
     for (size_t k = 1; k < cand.size(); ++k)
         if (tmin[k] < tmin[kb]) kb = k;
-    b->zlen = rc == NUOSC_OK ? cand[kb] : keep;
+    b->zlen = rc == NUOSC_OK ? cand[kb].first : keep;
+    b->launch = rc == NUOSC_OK ? cand[kb].second : keep_launch;
     b->dirty = true;
     batch_drop_graph(b);
     if (rc) return rc;
-    if (best_zlen) *best_zlen = cand[kb];
+    if (best_zlen) *best_zlen = cand[kb].first;
     if (best_ms) *best_ms = tmin[kb];
     return NUOSC_OK;
 }

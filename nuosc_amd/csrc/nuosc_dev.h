@@ -223,6 +223,12 @@ struct BatchMember {
     double *mpart, *mu, *ma, *mb;   // partial records; totals of u, yA, yB
     double dt;
 };
+// Launch forms of the batched stage kernel: an ordinary grid of one block
+// per logical block, or a persistent grid of resident blocks drawing logical
+// blocks from per-XCD work queues.
+constexpr int kBatchGrid = 0, kBatchPersistent = 1;
+constexpr int kBatchCounters = 32;   // 8 XCD counters for each of the 4 stages
+
 struct BatchTables {
     void* args[4] = {nullptr, nullptr, nullptr, nullptr};  // per-stage StageArgs arrays
     void* tot[4] = {nullptr, nullptr, nullptr, nullptr};   // per-stage moment-total arrays
@@ -230,9 +236,12 @@ struct BatchTables {
     int* bmap = nullptr;   // logical block -> (member, block of member)
     int* tmap = nullptr;
     int* xmap = nullptr;
+    int* ctr = nullptr;    // work-queue counters of the persistent launch
     int nblocks = 0, tblocks = 0, xblocks = 0, esize = 8;
+    int launch = kBatchGrid, pgrid = 0;   // launch form; resident blocks of the persistent grid
 };
-hipError_t batch_build(BatchTables& t, const std::vector<BatchMember>& members, int esize, hipStream_t s);
+hipError_t batch_build(BatchTables& t, const std::vector<BatchMember>& members, int esize, int launch,
+                       hipStream_t s);
 void batch_free(BatchTables& t);
 hipError_t batch_step(const BatchTables& t, hipStream_t s);
 // one stage kernel of every member (no ghost refresh, no moment totals):

@@ -806,6 +806,44 @@ __global__ __launch_bounds__(64 * StageShape<T>::W) __attribute__((amdgpu_waves_
     stage_body<T, R, W, 2, STAGE, COMP>(a, blk, wib, slab);
 }
 
+// The same work as a persistent kernel: a grid of resident blocks takes
+// logical blocks from a work queue until it is empty.  Logical blocks keep
+// the split of xcd_remap (one contiguous range per XCD, block b of the grid
+// runs on XCD b % 8), and each XCD's blocks draw from their range through
+// their own counter ctr[x], so neighbouring logical blocks still run on one
+// XCD at about the same time.  Every block leaves the loop once its XCD's
+// counter passes the range (the exit every wave reaches); the counters are
+// zeroed by k_zero_ints before each launch.
+template <typename T, int R, int STAGE, bool COMP>
+__global__ __launch_bounds__(64 * StageShape<T>::W) __attribute__((amdgpu_waves_per_eu(stage_waves<T, STAGE>()))) void k_stage_batch_q(
+    const StageArgs<T>* __restrict__ runs, const int* __restrict__ bmap, int* __restrict__ ctr, int nitems) {
+    constexpr int W = StageShape<T>::W;
+    __shared__ Slab<T, R, W, 2> slab;
+    __shared__ int next;
+    const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int x = blockIdx.x % 8;
+    const int per = nitems / 8, rem = nitems % 8;
+    const int start = x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per;
+    const int count = per + (x < rem ? 1 : 0);
+    const __attribute__((address_space(4))) int* cm = (const __attribute__((address_space(4))) int*)bmap;
+    for (;;) {
+        if (threadIdx.x == 0) next = atomicAdd(&ctr[x], 1);
+        __syncthreads();
+        const int k = __builtin_amdgcn_readfirstlane(next);
+        __syncthreads();   // every wave has read next before thread 0 rewrites it
+        if (k >= count) break;
+        const int lb = start + k;
+        const int run = cm[2 * lb], blk = cm[2 * lb + 1];
+        const StageArgs<T> a = runs[run];   // uniform, read-only: scalar loads
+        stage_body<T, R, W, 2, STAGE, COMP>(a, blk, wib, slab);
+    }
+}
+
+__global__ void k_zero_ints(int* __restrict__ p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
 int rows_per_wave(int esize) { return esize == 8 ? StageShape<double>::R : StageShape<float>::R; }
 int waves_per_block(int esize) { return esize == 8 ? StageShape<double>::W : StageShape<float>::W; }
 int dirs_per_chunk(int esize) {
@@ -1090,6 +1128,19 @@ static hipError_t batch_build_t(BatchTables& t, const std::vector<BatchMember>& 
     t.tblocks = (int)tmap.size() / 2;
     t.xblocks = (int)xmap.size() / 2;
     hipError_t e;
+    // persistent launch: one counter per XCD per stage, and a grid of as many
+    // blocks as the GPU holds at once (stage 2, the largest, sets the count)
+    if ((e = hipMalloc((void**)&t.ctr, kBatchCounters * sizeof(int))) != hipSuccess) return e;
+    {
+        int dev = 0, cus = 0, bpc = 0;
+        if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+        if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
+        constexpr int R = StageShape<T>::R;
+        if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_stage_batch_q<T, R, 2, sizeof(T) == 4>,
+                                                              64 * StageShape<T>::W, 0)) != hipSuccess)
+            return e;
+        t.pgrid = std::max(8, (cus * std::max(bpc, 1) + 7) / 8 * 8);
+    }
     for (auto [dst, v] : {std::pair<int**, std::vector<int>*>{&t.bmap, &bmap}, {&t.tmap, &tmap}, {&t.xmap, &xmap}}) {
         if ((e = hipMalloc((void**)dst, v->size() * sizeof(int))) != hipSuccess) return e;
         if ((e = hipMemcpyAsync(*dst, v->data(), v->size() * sizeof(int), hipMemcpyHostToDevice, s)) != hipSuccess)
@@ -1135,9 +1186,10 @@ static hipError_t batch_build_t(BatchTables& t, const std::vector<BatchMember>& 
     return hipStreamSynchronize(s);
 }
 
-hipError_t batch_build(BatchTables& t, const std::vector<BatchMember>& ms, int esize, hipStream_t s) {
+hipError_t batch_build(BatchTables& t, const std::vector<BatchMember>& ms, int esize, int launch, hipStream_t s) {
     batch_free(t);
     t.esize = esize;
+    t.launch = launch;
     if (ms.empty()) return hipSuccess;
     for (const auto& m : ms) {
         if (!buffer_addressable(m.g, esize)) return hipErrorInvalidValue;
@@ -1154,11 +1206,11 @@ void batch_free(BatchTables& t) {
         if (t.xg[k]) (void)hipFree(t.xg[k]);
         t.args[k] = t.tot[k] = t.xg[k] = nullptr;
     }
-    for (int** m : {&t.bmap, &t.tmap, &t.xmap}) {
+    for (int** m : {&t.bmap, &t.tmap, &t.xmap, &t.ctr}) {
         if (*m) (void)hipFree(*m);
         *m = nullptr;
     }
-    t.nblocks = t.tblocks = t.xblocks = 0;
+    t.nblocks = t.tblocks = t.xblocks = t.pgrid = 0;
 }
 
 template <typename T>
@@ -1167,6 +1219,19 @@ static hipError_t batch_stage_t(const BatchTables& t, int st, hipStream_t s) {
     constexpr bool COMP = sizeof(T) == 4;
     const dim3 grid(t.nblocks), block(64 * StageShape<T>::W);
     const StageArgs<T>* a = (const StageArgs<T>*)t.args[st - 1];
+    if (t.launch == kBatchPersistent) {
+        int* c = t.ctr + 8 * (st - 1);
+        hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, s, c, 8);   // a kernel node, not a memset, in graphs
+        const dim3 pg(t.pgrid);
+        const int n = t.nblocks;
+        switch (st) {
+            case 1: hipLaunchKernelGGL((k_stage_batch_q<T, R, 1, COMP>), pg, block, 0, s, a, t.bmap, c, n); break;
+            case 2: hipLaunchKernelGGL((k_stage_batch_q<T, R, 2, COMP>), pg, block, 0, s, a, t.bmap, c, n); break;
+            case 3: hipLaunchKernelGGL((k_stage_batch_q<T, R, 3, COMP>), pg, block, 0, s, a, t.bmap, c, n); break;
+            default: hipLaunchKernelGGL((k_stage_batch_q<T, R, 4, COMP>), pg, block, 0, s, a, t.bmap, c, n); break;
+        }
+        return hipGetLastError();
+    }
     switch (st) {
         case 1: hipLaunchKernelGGL((k_stage_batch<T, R, 1, COMP>), grid, block, 0, s, a, t.bmap); break;
         case 2: hipLaunchKernelGGL((k_stage_batch<T, R, 2, COMP>), grid, block, 0, s, a, t.bmap); break;

@@ -1,6 +1,7 @@
 """Batched runs: members with different grids, direction counts and physics
-advance together (one launch per RK4 stage for all) and every member's state
-is bitwise the one it reaches stepped alone; members can leave and join."""
+advance together (one launch per RK4 stage for all, as an ordinary grid or a
+persistent work-queue grid) and every member's state is bitwise the one it
+reaches stepped alone; members can leave and join."""
 import numpy as np
 import pytest
 
@@ -50,6 +51,40 @@ def test_batch_equals_alone(nuosc_mod, precision):
     mixed[1].step_rk4(5)
     alone[1].step_rk4(5)
     assert np.array_equal(alone[1].get_state(), mixed[1].get_state())
+    b.close()
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_batch_persistent_launch(nuosc_mod, precision):
+    """The persistent launch (resident blocks drawing logical blocks from
+    per-XCD work queues) gives every member bitwise the state it reaches
+    stepped alone, over replays of the captured step (the queue counters are
+    zeroed by a kernel node in each); the autotuner's (zlen, launch form)
+    choice keeps that, and an unknown form is refused."""
+    N = nuosc_mod
+    alone = [make(N, c, 200 + k, precision) for k, c in enumerate(RUNS)]
+    mixed = [make(N, c, 200 + k, precision) for k, c in enumerate(RUNS)]
+    b = N.Batch(precision=precision)
+    for g in mixed:
+        b.add(g)
+    assert b.launch == N.BATCH_GRID
+    b.set_launch(N.BATCH_PERSISTENT)
+    assert b.launch == N.BATCH_PERSISTENT
+    b.step(6)
+    for g in alone:
+        g.step_rk4(6)
+    for k, (a, m) in enumerate(zip(alone, mixed)):
+        assert np.array_equal(a.get_state(), m.get_state()), k
+    zl, ms = b.autotune(trials=1)
+    assert zl in (16, 32, 64, 128, 256) and ms > 0
+    assert b.launch in (N.BATCH_GRID, N.BATCH_PERSISTENT)
+    b.step(3)
+    for g in alone:
+        g.step_rk4(3)
+    for k, (a, m) in enumerate(zip(alone, mixed)):
+        assert np.array_equal(a.get_state(), m.get_state()), k
+    with pytest.raises(N.NuOscError):
+        b.set_launch(2)
     b.close()
 
 
