@@ -478,7 +478,15 @@ __device__ __forceinline__ void stage3z_body(const F3ZArgs<T>& a, ZSlab<T>& sl, 
     const int j0 = zs * a.zlen;
     const int j1 = min(nz, j0 + a.zlen);
     T win[NF][5];   // own fields, z = j-2 .. j+2
-    T hreg[NF];     // edge row of z = j+1 (written at step j+1)
+    // Edge rows.  FP64 ("early"): step j loads the edge row of z+2, at the
+    // time the neighbouring blocks load that row as their own, so it is
+    // mostly an L2 hit (HBM reads of the stage fell by 3-7 GB per launch),
+    // and step j+1 writes it after its barrier into the other buffer, the
+    // slab of step j+2.  FP32: step j loads the edge row of z+1 after its
+    // barrier and step j+1 writes it with its centre values (the early form
+    // spilled more VGPRs in FP32 and was 4% slower there).
+    constexpr bool kEdgeEarly = sizeof(T) == 8;
+    T hreg[NF];     // the edge row of z+1 at the top of step j
     {
         const __amdgpu_buffer_rsrc_t ry = rs(a.yin);
 #pragma unroll
@@ -488,35 +496,53 @@ __device__ __forceinline__ void stage3z_body(const F3ZArgs<T>& a, ZSlab<T>& sl, 
             for (int q = 0; q < NF; ++q)
                 win[q][qz] = zld<T>(ry, vo, so + (unsigned)kGroupField[grp][q] * rowb);
         }
-        const unsigned so = zrow(j0) * siteb;
+        const unsigned so = zrow(j0) * siteb, s1 = zrow(j0 + 1) * siteb;
+        if constexpr (kEdgeEarly) {   // z = j0 straight into the slab of step j0, z = j0+1 into hreg
 #pragma unroll
-        for (int q = 0; q < NF; ++q)
-            hreg[q] = zld<T>(ry, vh, so + (unsigned)kGroupField[grp][q] * rowb);
+            for (int q = 0; q < NF; ++q)
+                sl.x[j0 & 1][kGroupField[grp][q]][slot_h][lane] =
+                    zld<T>(ry, vh, so + (unsigned)kGroupField[grp][q] * rowb);
+#pragma unroll
+            for (int q = 0; q < NF; ++q) hreg[q] = zld<T>(ry, vh, s1 + (unsigned)kGroupField[grp][q] * rowb);
+        } else {
+#pragma unroll
+            for (int q = 0; q < NF; ++q) hreg[q] = zld<T>(ry, vh, so + (unsigned)kGroupField[grp][q] * rowb);
+        }
     }
     for (int j = j0; j < j1; ++j) {
         const unsigned so = (unsigned)(j + zoff) * siteb;
-        // ---- loads: own row z+2 (this step), edge row z+1 (next step)
+        // ---- loads: own row z+2 (this step); the edge row follows the barrier
         const __amdgpu_buffer_rsrc_t ry = rs(a.yin);
         const unsigned s2 = zrow(j + 2) * siteb, s1 = zrow(j + 1) * siteb;
 #pragma unroll
         for (int q = 0; q < NF; ++q)
             win[q][4] = zld<T>(ry, vo, s2 + (unsigned)kGroupField[grp][q] * rowb);
         __builtin_amdgcn_sched_barrier(0);
-        // ---- LDS: own centre values and the edge rows of z (loaded last step)
+        // ---- LDS: own centre values of z (early form: the edge rows of z are
+        // there already; else they are written now)
         T (*X)[kZSlots][64] = sl.x[j & 1];
 #pragma unroll
-        for (int q = 0; q < NF; ++q)
-            {
-                const int f = kGroupField[grp][q];
-                X[f][slot_c][lane] = win[q][2];
-                X[f][slot_h][lane] = hreg[q];
-            }
+        for (int q = 0; q < NF; ++q) {
+            const int f = kGroupField[grp][q];
+            X[f][slot_c][lane] = win[q][2];
+            if constexpr (!kEdgeEarly) X[f][slot_h][lane] = hreg[q];
+        }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
         __builtin_amdgcn_s_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+        if constexpr (kEdgeEarly) {
+            // every wave is done with step j-1, so the other buffer (the slab
+            // of step j+1) takes the edge rows of z+1, published by the next
+            // barrier; then the edge rows of z+2 are loaded
+            T (*Xn)[kZSlots][64] = sl.x[(j + 1) & 1];
 #pragma unroll
-        for (int q = 0; q < NF; ++q)
-            hreg[q] = zld<T>(ry, vh, s1 + (unsigned)kGroupField[grp][q] * rowb);
+            for (int q = 0; q < NF; ++q) Xn[kGroupField[grp][q]][slot_h][lane] = hreg[q];
+#pragma unroll
+            for (int q = 0; q < NF; ++q) hreg[q] = zld<T>(ry, vh, s2 + (unsigned)kGroupField[grp][q] * rowb);
+        } else {
+#pragma unroll
+            for (int q = 0; q < NF; ++q) hreg[q] = zld<T>(ry, vh, s1 + (unsigned)kGroupField[grp][q] * rowb);
+        }
         // ---- centre values of all 18 fields, moment projections, local terms
         T c[kNVar3];
 #pragma unroll
