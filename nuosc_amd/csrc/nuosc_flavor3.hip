@@ -486,7 +486,7 @@ __device__ __forceinline__ void stage3z_body(const F3ZArgs<T>& a, ZSlab<T>& sl, 
     // barrier and step j+1 writes it with its centre values (the early form
     // spilled more VGPRs in FP32 and was 4% slower there).
     constexpr bool kEdgeEarly = sizeof(T) == 8;
-    T hreg[NF];     // the edge row of z+1 at the top of step j
+    T hreg[NF];     // at the top of step j: the edge row of z+1 (early form) or of z
     {
         const __amdgpu_buffer_rsrc_t ry = rs(a.yin);
 #pragma unroll
