@@ -478,15 +478,17 @@ __device__ __forceinline__ void stage3z_body(const F3ZArgs<T>& a, ZSlab<T>& sl, 
     const int j0 = zs * a.zlen;
     const int j1 = min(nz, j0 + a.zlen);
     T win[NF][5];   // own fields, z = j-2 .. j+2
-    // Edge rows.  FP64 ("early"): step j loads the edge row of z+2, at the
-    // time the neighbouring blocks load that row as their own, so it is
-    // mostly an L2 hit (HBM reads of the stage fell by 3-7 GB per launch),
-    // and step j+1 writes it after its barrier into the other buffer, the
-    // slab of step j+2.  FP32: step j loads the edge row of z+1 after its
-    // barrier and step j+1 writes it with its centre values (the early form
-    // spilled more VGPRs in FP32 and was 4% slower there).
-    constexpr bool kEdgeEarly = sizeof(T) == 8;
-    T hreg[NF];     // at the top of step j: the edge row of z+1 (early form) or of z
+    // Edge rows: step j loads the edge row of z+2, in the step in which the
+    // neighbouring blocks load that row as their own, so it is mostly an L2
+    // hit (HBM reads of the stage fell by 3-7 GB per FP64 launch against a
+    // load one step later), and step j+1 writes it after its barrier into
+    // the other buffer, the slab of step j+2.  FP64 issues the load right
+    // after step j's barrier (latency hidden by the step's compute); FP32
+    // at the end of step j, which keeps the row in registers only until the
+    // next barrier (0-2 spilled VGPRs instead of 6-8: 14% faster per FP32
+    // step; the same in FP64 was 4.5% slower).
+    constexpr bool kEdgeAtEnd = sizeof(T) == 4;
+    T hreg[NF];     // at the top of step j: the edge row of z+1
     {
         const __amdgpu_buffer_rsrc_t ry = rs(a.yin);
 #pragma unroll
@@ -496,52 +498,42 @@ __device__ __forceinline__ void stage3z_body(const F3ZArgs<T>& a, ZSlab<T>& sl, 
             for (int q = 0; q < NF; ++q)
                 win[q][qz] = zld<T>(ry, vo, so + (unsigned)kGroupField[grp][q] * rowb);
         }
+        // z = j0 straight into the slab of step j0, z = j0+1 into hreg
         const unsigned so = zrow(j0) * siteb, s1 = zrow(j0 + 1) * siteb;
-        if constexpr (kEdgeEarly) {   // z = j0 straight into the slab of step j0, z = j0+1 into hreg
 #pragma unroll
-            for (int q = 0; q < NF; ++q)
-                sl.x[j0 & 1][kGroupField[grp][q]][slot_h][lane] =
-                    zld<T>(ry, vh, so + (unsigned)kGroupField[grp][q] * rowb);
+        for (int q = 0; q < NF; ++q)
+            sl.x[j0 & 1][kGroupField[grp][q]][slot_h][lane] = zld<T>(ry, vh, so + (unsigned)kGroupField[grp][q] * rowb);
 #pragma unroll
-            for (int q = 0; q < NF; ++q) hreg[q] = zld<T>(ry, vh, s1 + (unsigned)kGroupField[grp][q] * rowb);
-        } else {
-#pragma unroll
-            for (int q = 0; q < NF; ++q) hreg[q] = zld<T>(ry, vh, so + (unsigned)kGroupField[grp][q] * rowb);
-        }
+        for (int q = 0; q < NF; ++q) hreg[q] = zld<T>(ry, vh, s1 + (unsigned)kGroupField[grp][q] * rowb);
     }
     for (int j = j0; j < j1; ++j) {
         const unsigned so = (unsigned)(j + zoff) * siteb;
         // ---- loads: own row z+2 (this step); the edge row follows the barrier
         const __amdgpu_buffer_rsrc_t ry = rs(a.yin);
-        const unsigned s2 = zrow(j + 2) * siteb, s1 = zrow(j + 1) * siteb;
+        const unsigned s2 = zrow(j + 2) * siteb;
 #pragma unroll
         for (int q = 0; q < NF; ++q)
             win[q][4] = zld<T>(ry, vo, s2 + (unsigned)kGroupField[grp][q] * rowb);
         __builtin_amdgcn_sched_barrier(0);
-        // ---- LDS: own centre values of z (early form: the edge rows of z are
-        // there already; else they are written now)
+        // ---- LDS: own centre values of z (the edge rows of z are there)
         T (*X)[kZSlots][64] = sl.x[j & 1];
 #pragma unroll
-        for (int q = 0; q < NF; ++q) {
-            const int f = kGroupField[grp][q];
-            X[f][slot_c][lane] = win[q][2];
-            if constexpr (!kEdgeEarly) X[f][slot_h][lane] = hreg[q];
-        }
+        for (int q = 0; q < NF; ++q) X[kGroupField[grp][q]][slot_c][lane] = win[q][2];
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
         __builtin_amdgcn_s_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-        if constexpr (kEdgeEarly) {
+        {
             // every wave is done with step j-1, so the other buffer (the slab
             // of step j+1) takes the edge rows of z+1, published by the next
-            // barrier; then the edge rows of z+2 are loaded
+            // barrier; then (FP64) the edge rows of z+2 are loaded
             T (*Xn)[kZSlots][64] = sl.x[(j + 1) & 1];
 #pragma unroll
             for (int q = 0; q < NF; ++q) Xn[kGroupField[grp][q]][slot_h][lane] = hreg[q];
+            if constexpr (!kEdgeAtEnd) {
 #pragma unroll
-            for (int q = 0; q < NF; ++q) hreg[q] = zld<T>(ry, vh, s2 + (unsigned)kGroupField[grp][q] * rowb);
-        } else {
-#pragma unroll
-            for (int q = 0; q < NF; ++q) hreg[q] = zld<T>(ry, vh, s1 + (unsigned)kGroupField[grp][q] * rowb);
+                for (int q = 0; q < NF; ++q)
+                    hreg[q] = zld<T>(ry, vh, s2 + (unsigned)kGroupField[grp][q] * rowb);
+            }
         }
         // ---- centre values of all 18 fields, moment projections, local terms
         T c[kNVar3];
@@ -625,6 +617,10 @@ __device__ __forceinline__ void stage3z_body(const F3ZArgs<T>& a, ZSlab<T>& sl, 
             const double tot = wave_sum16(t);
             if (row_ok && (lane & 3) == 0)
                 a.part[(((long long)i * nz + j) * a.nch + ch) * kNMom3 + rec_pos(grp, lane >> 2)] = tot;
+        }
+        if constexpr (kEdgeAtEnd) {   // (FP32) the edge rows of z+2, written after the next barrier
+#pragma unroll
+            for (int q = 0; q < NF; ++q) hreg[q] = zld<T>(ry, vh, s2 + (unsigned)kGroupField[grp][q] * rowb);
         }
         // no second barrier: step j+1 writes the other buffer, and step j+2's
         // writes to this one follow step j+1's barrier, which every wave
